@@ -1,0 +1,227 @@
+"""Benchmark of the SPH step engine on one MI355X.
+
+Default workload: the 1,048,576-particle double dam break (two jittered
+128x64x64 blocks, box [0,2]x[0,1]x[0,2], r 0.005, h 0.02, dt 1e-4), FP32.
+A step = hash (fused into the previous step's integrate), radix sort, reorder
++ cell table, density + EOS, pressure/viscosity force + integrate + clamp.
+
+Prints ONE JSON line.  `value` is particle-steps/s over K graph-replayed
+steps with the state resident in HBM.  `roofline` is the dominant kernel's
+algorithmic bytes per launch over its mean duration, measured with HIP events
+on the engine's stream in a second run of K steps; `cpu_baseline` times the
+oracle's OpenMP C++ step on this host over a bounded sample.  Extras:
+`batched` (256-scene ensemble, one launch per pass, vs the same scenes
+launched one engine at a time) and `adjoint` (rollout reverse pass vs
+forward wall-clock, 16 x 32,768 particles, 200 steps, checkpoint every 20).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "particle-steps/s @1M (plain vs batched); % HBM roofline; adjoint/fwd ratio"
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes per particle and pass (FP32, unpadded float3): hash 20,
+# sort 16 (one ideal pass), reorder 52, cell table 4 + 8 per occupied cell,
+# density 16, force 40, integrate + clamp 60.  The engine fuses hash into the
+# integrate pass and integrate into the force kernel.
+B_ALG = dict(hash=20.0, sort=16.0, reorder=56.0, density=16.0, force=100.0)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def time_steps(e, k):
+    e.synchronize()
+    t0 = time.perf_counter()
+    e.advance_steps(k)
+    e.synchronize()
+    return time.perf_counter() - t0
+
+
+def plain(args):
+    from simplefluid_amd import configs, engine
+    cfg = configs.double_dambreak_1m()
+    n = cfg["pos"].shape[0]
+    e = engine.Engine(cfg["params"])
+    e.set_particles(cfg["pos"])
+    e.make_ready()
+    if not args.no_autotune:
+        e.autotune()
+    log("tuning", e.get_tuning())
+    e.advance_steps(args.warmup)
+    e.synchronize()
+    t = time_steps(e, args.steps)
+    value = n * args.steps / t
+    log(f"plain: {n} particles, {args.steps} steps in {t:.4f}s -> {value:.4e} particle-steps/s")
+    # Per-pass durations: K more steps with HIP events between passes.
+    e.profile_enable(True)
+    e.advance_steps(args.steps)
+    ms, nst = e.profile_read()
+    e.profile_enable(False)
+    per = {k: v / max(nst, 1) for k, v in ms.items()}
+    log("ms per pass", {k: round(v, 5) for k, v in per.items()})
+    s, en = e.debug_cells()
+    occ = int((en > 0).sum())
+    b_step = sum(v for k, v in B_ALG.items() if k != "hash") + 20.0 * 0 + 8.0 * occ / n
+    dom = max(("sort", "reorder", "density", "force"), key=lambda k: per[k])
+    dom_bytes = B_ALG[dom] * n + (8.0 * occ if dom == "reorder" else 0.0)
+    achieved = dom_bytes / (per[dom] * 1e-3) / 1e9
+    traffic = load_traffic(dom)
+    roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic}
+    step_ms = t / args.steps * 1e3
+    step_roof = b_step * value / 1e9
+    extra = {"step_bytes_alg_per_particle": round(b_step, 2),
+             "step_achieved_GBs": round(step_roof, 1),
+             "step_frac": round(step_roof / HBM_PEAK_GBS, 4),
+             "pass_ms": {k: round(v, 5) for k, v in per.items()},
+             "occupied_cells": occ, "tuning": e.get_tuning()}
+    e.close()
+    return value, step_ms, roof, extra, cfg
+
+
+def load_traffic(kernel):
+    """HBM bytes per launch from the committed rocprofv3 counter summary."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        d = json.load(open(path))
+        return d.get(kernel, {}).get("bytes_per_launch")
+    except Exception:
+        return None
+
+
+def cpu_baseline(cfg, budget_s):
+    import oracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+    o = oracle.Oracle(oracle.from_params(cfg["params"]), threads=threads)
+    o.set_state(cfg["pos"].astype(np.float64))
+    o.step(1)  # warm-up
+    steps, spent = 0, 0.0
+    while spent < budget_s and steps < 50:
+        spent += o.step(1)
+        steps += 1
+    n = cfg["pos"].shape[0]
+    return {"value": round(n * steps / spent, 1), "unit": "particle-steps/s", "cores": threads,
+            "kind": "port",
+            "sample": f"oracle C++ FP32 step (OpenMP), {n} particles of the same workload, {steps} steps "
+                      f"after 1 warm-up, {spent:.1f}s"}
+
+
+def batched(args):
+    from simplefluid_amd import configs, engine
+    ens = configs.ensemble(nscenes=256, seed=3)
+    params = [s["params"] for s in ens["scenes"]]
+    counts = [s["pos"].shape[0] for s in ens["scenes"]]
+    N = int(sum(counts))
+    b = engine.BatchEngine(params, counts)
+    b.set_particles(np.concatenate([s["pos"] for s in ens["scenes"]]))
+    b.make_ready()
+    if not args.no_autotune:
+        b.autotune()
+    b.advance_steps(5)
+    k = args.batch_steps
+    tb = time_steps(b, k)
+    vb = N * k / tb
+    b.close()
+    # Per-scene launches: one engine per scene, stepped scene by scene.
+    engines = []
+    for s in ens["scenes"]:
+        e = engine.Engine(s["params"])
+        e.set_particles(s["pos"])
+        e.make_ready()
+        e.advance_steps(2)
+        engines.append(e)
+    for e in engines:
+        e.synchronize()
+    t0 = time.perf_counter()
+    for e in engines:
+        e.advance_steps(k)
+    for e in engines:
+        e.synchronize()
+    ts = time.perf_counter() - t0
+    vs = N * k / ts
+    for e in engines:
+        e.close()
+    log(f"batched: {N} particles in 256 scenes, {k} steps: {vb:.4e} vs per-scene {vs:.4e}")
+    return {"scenes": 256, "particles": N, "steps": k, "value": round(vb, 1),
+            "per_scene_value": round(vs, 1), "speedup": round(vb / vs, 3),
+            "unit": "particle-steps/s"}
+
+
+def adjoint(args):
+    from simplefluid_amd import configs, engine
+    rc = configs.rollout_scenes()
+    sc = rc["scenes"]
+    counts = [s["pos"].shape[0] for s in sc]
+    b = engine.BatchEngine([s["params"] for s in sc], counts)
+    pos0 = np.concatenate([s["pos"] for s in sc])
+    b.set_particles(pos0)
+    b.make_ready()
+    b.rollout_forward(10, 5)  # warm-up of both passes
+    b.rollout_backward(np.zeros((len(pos0), 3), np.float32))
+    b.set_particles(pos0)
+    b.make_ready()
+    b.synchronize()
+    t0 = time.perf_counter()
+    b.rollout_forward(rc["steps"], rc["checkpoint_every"])
+    b.synchronize()
+    tf = time.perf_counter() - t0
+    xT = b.get_particles()
+    g = (xT * 1e-3).astype(np.float32)
+    t0 = time.perf_counter()
+    b.rollout_backward(g)
+    tb = time.perf_counter() - t0
+    b.close()
+    log(f"adjoint: forward {tf:.4f}s backward {tb:.4f}s ratio {tb / tf:.3f}")
+    return {"scenes": len(sc), "particles": int(sum(counts)), "steps": rc["steps"],
+            "checkpoint_every": rc["checkpoint_every"], "forward_s": round(tf, 5),
+            "backward_s": round(tb, 5), "ratio": round(tb / tf, 3)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--batch-steps", type=int, default=100)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported by this build "
+              "(scenes shard with no communication; see the design notes)", file=sys.stderr)
+        sys.exit(2)
+    value, step_ms, roof, extra, cfg = plain(args)
+    out = {"metric": METRIC, "value": round(value, 1), "unit": "particle-steps/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(step_ms, 5),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic",
+           "config": {"workload": "double_dambreak_1m", "particles": int(cfg["pos"].shape[0]),
+                      "box": [2.0, 1.0, 2.0], "particle_radius": 0.005, "kernel_radius": 0.02,
+                      "dt": 1e-4, "stiffness": 50.0, "viscosity": 0.01},
+           "roofline": roof, "step": extra}
+    if not args.no_extras:
+        out["batched"] = batched(args)
+        out["adjoint"] = adjoint(args)
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,372 @@
+// TEST INFRASTRUCTURE ONLY. Not part of the product and never linked into it.
+//
+// CPU restatement of one weakly-compressible SPH step, used as the parity
+// checker for the HIP engine and as the timed CPU baseline.
+//
+// The reference (ttnghia/SimpleFluid) drives its solver through
+//   Source/Simulator.cpp:42   m_SPHSolver->makeReady();
+//   Source/Simulator.cpp:49   frameTime += m_SPHSolver->advanceFrame();
+// but the solver itself (Banana ParticleSolvers/SPH/SPHSolver.h, included at
+// Include/QtSPHSolver.h:22) is not vendored, so its arithmetic is unknown.
+// This file restates the formulation the engine defines instead:
+//   * support h = kernelRadius (Source/Controller.cpp:55: kernelRadius = 2/res),
+//     particle radius r = h/4, lattice spacing 2r (Source/SceneManager.cpp:45);
+//   * mass m = rho0 (2r)^3;
+//   * rho_i = m sum_j W_poly6(|x_i - x_j|), self term included (Mueller 2003);
+//   * Tait EOS p = B((rho/rho0)^7 - 1), p < 0 clamped to 0 unless the
+//     attractive-pressure option (Source/Controller.cpp:61) scales it;
+//   * symmetric pressure force with the spiky gradient, Mueller viscosity
+//     (mu/rho_i) sum m (v_j - v_i)/rho_j lap W_visc, gravity;
+//   * symplectic Euler and a box clamp with restitution on the normal velocity.
+// Every op sequence that decides a discrete outcome (cell index, neighbour
+// predicate) is written exactly as the engine writes it, so cells and
+// neighbour sets are comparable bit for bit.  Float sums run in the canonical
+// neighbour order: rows dz = -1..1 (outer), dy = -1..1 (inner), each row the
+// contiguous sorted range of cells cx-1..cx+1, ascending sorted index.
+#pragma once
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <numeric>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace orc {
+
+// Same field meaning as the engine's sph_params.
+struct Params {
+    float particle_radius;
+    float kernel_radius;
+    float rest_density;
+    float pressure_stiffness;
+    float viscosity;
+    float boundary_restitution;
+    int   use_attractive_pressure;
+    float attractive_pressure_ratio;
+    float gravity[3];
+    float dt;
+    float box_min[3];
+    float box_max[3];
+};
+
+template <class T>
+struct Consts {
+    T h, h2, inv_cs, mass, cD, cP, cV, inv_rho0, B, e, attr, dt, r;
+    T g[3], bmin[3], lo[3], hi[3];
+    int G[3];
+    bool attractive;
+    uint32_t ncells;
+};
+
+// Derived constants. Kernel normalisations are formed in double from the
+// float inputs and rounded once to T.  Cell size is h(1 + 2^-10): a pair
+// closer than h can then never land two cells apart through rounding of
+// (x - box_min) * inv_cs for grids up to 1024 cells per axis.
+template <class T>
+inline Consts<T> derive(const Params& p) {
+    Consts<T> c{};
+    const T r   = (T)p.particle_radius;
+    const T sp  = T(2) * r;
+    c.r         = r;
+    c.mass      = (T)p.rest_density * (sp * sp * sp);
+    c.h         = (T)p.kernel_radius;
+    c.h2        = c.h * c.h;
+    const double hd = (double)(T)p.kernel_radius;
+    const double h3 = hd * hd * hd, h6 = h3 * h3, h9 = h6 * h3;
+    const double poly6 = 315.0 / (64.0 * M_PI * h9);
+    const double spiky = 45.0 / (M_PI * h6);
+    c.cD        = (T)((double)c.mass * poly6);
+    c.cP        = (T)((double)c.mass * spiky);
+    c.cV        = (T)((double)(T)p.viscosity * (double)c.mass * spiky);
+    c.inv_rho0  = (T)(1.0 / (double)(T)p.rest_density);
+    c.inv_cs    = (T)(1.0 / (hd * (1.0 + 1.0 / 1024.0)));
+    c.B         = (T)p.pressure_stiffness;
+    c.e         = (T)p.boundary_restitution;
+    c.attr      = (T)p.attractive_pressure_ratio;
+    c.attractive = p.use_attractive_pressure != 0;
+    c.dt        = (T)p.dt;
+    c.ncells    = 1;
+    for (int k = 0; k < 3; ++k) {
+        c.g[k]    = (T)p.gravity[k];
+        c.bmin[k] = (T)p.box_min[k];
+        const T ext = (T)p.box_max[k] - (T)p.box_min[k];
+        int G = (int)std::ceil(ext * c.inv_cs);
+        c.G[k] = G < 1 ? 1 : G;
+        c.lo[k] = (T)p.box_min[k] + r;
+        c.hi[k] = (T)p.box_max[k] - r;
+        c.ncells *= (uint32_t)c.G[k];
+    }
+    return c;
+}
+
+template <class T>
+inline int cell_coord(T x, T bmin, T inv_cs, int G) {
+    T f = (x - bmin) * inv_cs;
+    f = std::fmin(std::fmax(f, T(0)), (T)(G - 1));
+    return (int)f;
+}
+
+template <class T>
+inline uint32_t cell_key(const Consts<T>& c, T x, T y, T z) {
+    const int cx = cell_coord(x, c.bmin[0], c.inv_cs, c.G[0]);
+    const int cy = cell_coord(y, c.bmin[1], c.inv_cs, c.G[1]);
+    const int cz = cell_coord(z, c.bmin[2], c.inv_cs, c.G[2]);
+    return ((uint32_t)cz * (uint32_t)c.G[1] + (uint32_t)cy) * (uint32_t)c.G[0] + (uint32_t)cx;
+}
+
+template <class T>
+inline T dist2(T dx, T dy, T dz) {
+    return std::fma(dz, dz, std::fma(dy, dy, dx * dx));
+}
+
+// Tait EOS with exponent 7 by a fixed multiply chain: x^2, x^4, (x^4 x^2) x.
+template <class T>
+inline T eos(const Consts<T>& c, T rho) {
+    const T x  = rho * c.inv_rho0;
+    const T x2 = x * x;
+    const T x4 = x2 * x2;
+    const T x7 = (x4 * x2) * x;
+    T p = c.B * (x7 - T(1));
+    if (p < T(0)) p = c.attractive ? p * c.attr : T(0);
+    return p;
+}
+
+template <class T>
+struct Sim {
+    Params   prm{};
+    Consts<T> c{};
+    uint32_t n = 0;
+    // State, in the engine's state order (the sorted order of the last step).
+    std::vector<T> x, y, z, vx, vy, vz;
+    std::vector<uint32_t> id;
+    // Intermediates of the last step.
+    std::vector<uint32_t> key;         // state order, key of the input state
+    std::vector<uint32_t> perm;        // sorted index -> state index
+    std::vector<uint32_t> skey;        // sorted keys
+    std::vector<uint32_t> cstart, cend; // dense cell table, empty = (~0u, 0)
+    std::vector<T> sx, sy, sz, svx, svy, svz;  // reordered input state
+    std::vector<T> rho, pterm, irho, ax, ay, az; // sorted order
+    std::vector<uint32_t> noff, nbr;   // neighbour CSR in sorted index space
+    bool record_nbrs = false;
+    int nthreads = 1;
+
+    void set_params(const Params& p) {
+        prm = p;
+        c = derive<T>(p);
+    }
+
+    void set_state(uint32_t count, const T* pos3, const T* vel3, const uint32_t* ids) {
+        n = count;
+        x.resize(n); y.resize(n); z.resize(n); vx.resize(n); vy.resize(n); vz.resize(n); id.resize(n);
+        for (uint32_t i = 0; i < n; ++i) {
+            x[i] = pos3[3 * i]; y[i] = pos3[3 * i + 1]; z[i] = pos3[3 * i + 2];
+            vx[i] = vel3 ? vel3[3 * i] : T(0);
+            vy[i] = vel3 ? vel3[3 * i + 1] : T(0);
+            vz[i] = vel3 ? vel3[3 * i + 2] : T(0);
+            id[i] = ids ? ids[i] : i;
+        }
+    }
+
+    // Stable counting sort of the state indices by key.  Chunks are counted
+    // per thread and offsets laid out cell-major, thread-minor, so the result
+    // equals a serial stable sort.
+    void sort_keys() {
+        const uint32_t C = c.ncells;
+        perm.assign(n, 0);
+        skey.assign(n, 0);
+        int T_ = 1;
+#ifdef _OPENMP
+        T_ = std::min(16, std::max(1, nthreads));  // T_ x C counters
+#endif
+        if (T_ == 1 || n < 65536) {
+            std::vector<uint32_t> cnt((size_t)C + 1, 0);
+            for (uint32_t i = 0; i < n; ++i) cnt[key[i] + 1]++;
+            for (uint32_t k = 0; k < C; ++k) cnt[k + 1] += cnt[k];
+            for (uint32_t i = 0; i < n; ++i) {
+                const uint32_t d = cnt[key[i]]++;
+                perm[d] = i;
+                skey[d] = key[i];
+            }
+            return;
+        }
+        std::vector<uint32_t> cnt((size_t)T_ * C, 0);
+        const uint32_t chunk = (n + T_ - 1) / T_;
+#pragma omp parallel num_threads(T_)
+        {
+            int t = 0;
+#ifdef _OPENMP
+            t = omp_get_thread_num();
+#endif
+            const uint32_t b = std::min(n, chunk * t), e = std::min(n, b + chunk);
+            uint32_t* my = &cnt[(size_t)t * C];
+            for (uint32_t i = b; i < e; ++i) my[key[i]]++;
+#pragma omp barrier
+#pragma omp single
+            {
+                uint32_t run = 0;
+                for (uint32_t k = 0; k < C; ++k)
+                    for (int tt = 0; tt < T_; ++tt) {
+                        uint32_t& v = cnt[(size_t)tt * C + k];
+                        const uint32_t cnt_v = v;
+                        v = run;
+                        run += cnt_v;
+                    }
+            }
+            for (uint32_t i = b; i < e; ++i) {
+                const uint32_t d = my[key[i]]++;
+                perm[d] = i;
+                skey[d] = key[i];
+            }
+        }
+    }
+
+    void row_range(int cx, int y_, int z_, uint32_t& lo, uint32_t& hi) const {
+        lo = ~0u; hi = 0;
+        if (y_ < 0 || y_ >= c.G[1] || z_ < 0 || z_ >= c.G[2]) return;
+        const int x0 = std::max(cx - 1, 0), x1 = std::min(cx + 1, c.G[0] - 1);
+        const uint32_t base = ((uint32_t)z_ * (uint32_t)c.G[1] + (uint32_t)y_) * (uint32_t)c.G[0];
+        for (int xx = x0; xx <= x1; ++xx) {
+            lo = std::min(lo, cstart[base + xx]);
+            hi = std::max(hi, cend[base + xx]);
+        }
+    }
+
+    void step() {
+        const int nt = std::max(1, nthreads);
+        (void)nt;
+        key.resize(n);
+#pragma omp parallel for num_threads(nt) schedule(static)
+        for (long long i = 0; i < (long long)n; ++i) key[i] = cell_key(c, x[i], y[i], z[i]);
+        sort_keys();
+        // Reorder.
+        sx.resize(n); sy.resize(n); sz.resize(n); svx.resize(n); svy.resize(n); svz.resize(n);
+        std::vector<uint32_t> sid(n);
+#pragma omp parallel for num_threads(nt) schedule(static)
+        for (long long k = 0; k < (long long)n; ++k) {
+            const uint32_t s = perm[k];
+            sx[k] = x[s]; sy[k] = y[s]; sz[k] = z[s];
+            svx[k] = vx[s]; svy[k] = vy[s]; svz[k] = vz[s];
+            sid[k] = id[s];
+        }
+        // Cell table.
+        cstart.assign(c.ncells, ~0u);
+        cend.assign(c.ncells, 0u);
+        for (uint32_t k = 0; k < n; ++k) {
+            if (k == 0 || skey[k - 1] != skey[k]) cstart[skey[k]] = k;
+            if (k + 1 == n || skey[k + 1] != skey[k]) cend[skey[k]] = k + 1;
+        }
+        // Density + EOS.
+        rho.resize(n); pterm.resize(n); irho.resize(n);
+        const T h2 = c.h2;
+#pragma omp parallel for num_threads(nt) schedule(static)
+        for (long long k = 0; k < (long long)n; ++k) {
+            const T xi = sx[k], yi = sy[k], zi = sz[k];
+            const int cx = cell_coord(xi, c.bmin[0], c.inv_cs, c.G[0]);
+            const int cy = cell_coord(yi, c.bmin[1], c.inv_cs, c.G[1]);
+            const int cz = cell_coord(zi, c.bmin[2], c.inv_cs, c.G[2]);
+            T sum = 0;
+            for (int dz = -1; dz <= 1; ++dz)
+                for (int dy = -1; dy <= 1; ++dy) {
+                    uint32_t lo, hi;
+                    row_range(cx, cy + dy, cz + dz, lo, hi);
+                    for (uint32_t j = lo; j < hi; ++j) {
+                        const T d2 = dist2(xi - sx[j], yi - sy[j], zi - sz[j]);
+                        if (d2 < h2) {
+                            const T t = h2 - d2;
+                            sum = std::fma(t * t, t, sum);
+                        }
+                    }
+                }
+            const T r = c.cD * sum;
+            rho[k] = r;
+            const T p = eos(c, r);
+            const T ir = T(1) / r;
+            irho[k] = ir;
+            pterm[k] = (p * ir) * ir;
+        }
+        // Neighbour CSR (test only).
+        if (record_nbrs) {
+            noff.assign((size_t)n + 1, 0);
+            std::vector<std::vector<uint32_t>> tmp(n);
+#pragma omp parallel for num_threads(nt) schedule(static)
+            for (long long k = 0; k < (long long)n; ++k) {
+                const T xi = sx[k], yi = sy[k], zi = sz[k];
+                const int cx = cell_coord(xi, c.bmin[0], c.inv_cs, c.G[0]);
+                const int cy = cell_coord(yi, c.bmin[1], c.inv_cs, c.G[1]);
+                const int cz = cell_coord(zi, c.bmin[2], c.inv_cs, c.G[2]);
+                for (int dz = -1; dz <= 1; ++dz)
+                    for (int dy = -1; dy <= 1; ++dy) {
+                        uint32_t lo, hi;
+                        row_range(cx, cy + dy, cz + dz, lo, hi);
+                        for (uint32_t j = lo; j < hi; ++j)
+                            if (dist2(xi - sx[j], yi - sy[j], zi - sz[j]) < h2) tmp[k].push_back(j);
+                    }
+            }
+            for (uint32_t k = 0; k < n; ++k) noff[k + 1] = noff[k] + (uint32_t)tmp[k].size();
+            nbr.resize(noff[n]);
+            for (uint32_t k = 0; k < n; ++k) std::copy(tmp[k].begin(), tmp[k].end(), nbr.begin() + noff[k]);
+        }
+        // Forces + integration + clamp.
+        ax.resize(n); ay.resize(n); az.resize(n);
+        std::vector<T> nx(n), ny(n), nz(n), nvx(n), nvy(n), nvz(n);
+#pragma omp parallel for num_threads(nt) schedule(static)
+        for (long long k = 0; k < (long long)n; ++k) {
+            const T xi = sx[k], yi = sy[k], zi = sz[k];
+            const T vxi = svx[k], vyi = svy[k], vzi = svz[k];
+            const T pti = pterm[k];
+            const int cx = cell_coord(xi, c.bmin[0], c.inv_cs, c.G[0]);
+            const int cy = cell_coord(yi, c.bmin[1], c.inv_cs, c.G[1]);
+            const int cz = cell_coord(zi, c.bmin[2], c.inv_cs, c.G[2]);
+            T apx = 0, apy = 0, apz = 0, avx = 0, avy = 0, avz = 0;
+            for (int dz = -1; dz <= 1; ++dz)
+                for (int dy = -1; dy <= 1; ++dy) {
+                    uint32_t lo, hi;
+                    row_range(cx, cy + dy, cz + dz, lo, hi);
+                    for (uint32_t j = lo; j < hi; ++j) {
+                        if (j == (uint32_t)k) continue;
+                        const T dx = xi - sx[j], dy_ = yi - sy[j], dz_ = zi - sz[j];
+                        const T d2 = dist2(dx, dy_, dz_);
+                        if (!(d2 < h2)) continue;
+                        const T inv_r = d2 > T(0) ? T(1) / std::sqrt(d2) : T(0);
+                        const T rr = d2 * inv_r;
+                        const T hr = c.h - rr;
+                        const T w = hr * hr;
+                        const T s = ((pti + pterm[j]) * w) * inv_r;
+                        apx = std::fma(s, dx, apx);
+                        apy = std::fma(s, dy_, apy);
+                        apz = std::fma(s, dz_, apz);
+                        const T vv = hr * irho[j];
+                        avx = std::fma(vv, svx[j] - vxi, avx);
+                        avy = std::fma(vv, svy[j] - vyi, avy);
+                        avz = std::fma(vv, svz[j] - vzi, avz);
+                    }
+                }
+            const T cvi = c.cV * irho[k];
+            const T a0 = (c.cP * apx + cvi * avx) + c.g[0];
+            const T a1 = (c.cP * apy + cvi * avy) + c.g[1];
+            const T a2 = (c.cP * apz + cvi * avz) + c.g[2];
+            ax[k] = a0; ay[k] = a1; az[k] = a2;
+            T v[3] = {vxi + c.dt * a0, vyi + c.dt * a1, vzi + c.dt * a2};
+            T p[3] = {xi + c.dt * v[0], yi + c.dt * v[1], zi + c.dt * v[2]};
+            for (int d = 0; d < 3; ++d) {
+                if (p[d] < c.lo[d]) {
+                    p[d] = c.lo[d];
+                    if (v[d] < T(0)) v[d] = -c.e * v[d];
+                } else if (p[d] > c.hi[d]) {
+                    p[d] = c.hi[d];
+                    if (v[d] > T(0)) v[d] = -c.e * v[d];
+                }
+            }
+            nx[k] = p[0]; ny[k] = p[1]; nz[k] = p[2];
+            nvx[k] = v[0]; nvy[k] = v[1]; nvz[k] = v[2];
+        }
+        x.swap(nx); y.swap(ny); z.swap(nz); vx.swap(nvx); vy.swap(nvy); vz.swap(nvz);
+        id.swap(sid);
+    }
+};
+
+}  // namespace orc

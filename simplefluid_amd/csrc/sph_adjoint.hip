@@ -1,0 +1,250 @@
+// Hand-written reverse pass of one SPH step (gather form, no float atomics).
+//
+// Forward, per sorted particle i with neighbours j != i, r = x_i - x_j,
+// d = |r|, hr = h - d, t = h^2 - d^2, P = p/rho^2, q = 1/rho:
+//   rho_i = cD sum_j t^3                          (self included)
+//   a_i   = cP sum (P_i+P_j) hr^2/d r + cV q_i sum q_j hr (v_j - v_i) + g
+//   v' = v + dt a ; x~ = x + dt v' ; clamp(x~, v') -> (x', v'')
+// Reverse, given xbar', vbar'' (output order = this step's sorted order):
+//   adj_integrate: clamp masks, xbar~ , vbar' = vbar'' (or -e vbar'' where the
+//                  velocity was reflected) + dt xbar~, abar = dt vbar'
+//   adj_force:     xbar_i += sum cP(P_i+P_j)[phi da + (phi'/d)(da.r) r]
+//                           + cV q_i q_j (psi'/d)(da.dv) r
+//                  vbar_i += sum cV q_i q_j hr (abar_j - abar_i)
+//                  Pbar_i = sum cP phi da.r ; qbar_i = sum cV q_j hr da.dv
+//                  rhobar_i = Pbar_i (p' q^2 - 2 p q^3) - qbar_i q^2
+//                  with da = abar_i - abar_j, dv = v_j - v_i, phi = hr^2/d,
+//                  phi'/d = -t/d^3, psi' = -1; per-particle dL/dB and dL/dmu terms
+//   adj_density:   xbar_i += sum -6 cD t^2 (rhobar_i + rhobar_j) r, then
+//                  scatter both adjoints back to the step's input order.
+// Each kernel reads neighbour values written by the previous kernel only.
+#include "sph_internal.hpp"
+
+namespace sph {
+
+namespace {
+
+__device__ __forceinline__ void tile_span(uint32_t ntiles, uint32_t& t0, uint32_t& t1) {
+    const uint32_t lb = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t per = (ntiles + gridDim.x - 1) / gridDim.x;
+    t0 = lb * per;
+    t1 = t0 + per < ntiles ? t0 + per : ntiles;
+}
+
+// Visit the neighbours j != k the forward force pass visited, in its order.
+template <class F>
+__device__ __forceinline__ void for_each_neighbor(const AdjArgs& a, const SceneConst& sc, uint32_t k,
+                                                  const float4& pi, F&& f) {
+    const int cx = cell_coord(pi.x, sc.bmin[0], sc.inv_cs, sc.G[0]);
+    const int cy = cell_coord(pi.y, sc.bmin[1], sc.inv_cs, sc.G[1]);
+    const int cz = cell_coord(pi.z, sc.bmin[2], sc.inv_cs, sc.G[2]);
+#pragma unroll
+    for (int row = 0; row < kRows; ++row) {
+        const int dz = row / 3 - 1, dy = row % 3 - 1;
+        uint32_t lo, hi;
+        row_range(a.cells, sc, cx, cy + dy, cz + dz, lo, hi);
+        uint32_t m = a.mask[(size_t)row * a.n + k];
+        if (row == 4) {
+            const uint32_t b = k - lo;
+            if (b < (uint32_t)kMaskBits) m &= ~(1u << b);
+        }
+        while (m) {
+            const uint32_t b = (uint32_t)__builtin_ctz(m);
+            m &= m - 1u;
+            f(lo + b);
+        }
+        if (hi > lo + (uint32_t)kMaskBits)
+            for (uint32_t j = lo + (uint32_t)kMaskBits; j < hi; ++j) {
+                if (j == k) continue;
+                const float4 pj = a.pos_s[j];
+                if (dist2(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z) < sc.h2) f(j);
+            }
+    }
+}
+
+__global__ void adj_integrate_kernel(AdjArgs a) {
+    uint32_t t0, t1;
+    tile_span(a.ntiles, t0, t1);
+    for (uint32_t t = t0; t < t1; ++t) {
+        const uint2 ti = a.tiles[t];
+        const SceneConst& sc = a.scenes[ti.x];
+        const uint32_t k = ti.y + threadIdx.x;
+        if (k >= sc.p_end) continue;
+        const float4 x0 = a.pos_s[k], v0 = a.vel_s[k], ac = a.acc[k];
+        const float4 xb = a.xbar[k], vb = a.vbar[k];
+        const float xa[3] = {x0.x, x0.y, x0.z}, va[3] = {v0.x, v0.y, v0.z}, aa[3] = {ac.x, ac.y, ac.z};
+        const float xba[3] = {xb.x, xb.y, xb.z}, vba[3] = {vb.x, vb.y, vb.z};
+        float xo[3], vo[3], ab[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const float v1 = va[d] + sc.dt * aa[d];
+            const float xt = xa[d] + sc.dt * v1;
+            float xbt = xba[d], vb1 = vba[d];
+            if (xt < sc.lo[d]) {
+                xbt = 0.0f;
+                if (v1 < 0.0f) vb1 = -sc.e * vb1;
+            } else if (xt > sc.hi[d]) {
+                xbt = 0.0f;
+                if (v1 > 0.0f) vb1 = -sc.e * vb1;
+            }
+            vb1 += sc.dt * xbt;
+            xo[d] = xbt;
+            vo[d] = vb1;
+            ab[d] = sc.dt * vb1;
+        }
+        a.xs[k] = make_float4(xo[0], xo[1], xo[2], 0.0f);
+        a.vs[k] = make_float4(vo[0], vo[1], vo[2], 0.0f);
+        a.abar[k] = make_float4(ab[0], ab[1], ab[2], 0.0f);
+    }
+}
+
+__global__ void adj_force_kernel(AdjArgs a) {
+    uint32_t t0, t1;
+    tile_span(a.ntiles, t0, t1);
+    for (uint32_t t = t0; t < t1; ++t) {
+        const uint2 ti = a.tiles[t];
+        const SceneConst& sc = a.scenes[ti.x];
+        const uint32_t k = ti.y + threadIdx.x;
+        if (k >= sc.p_end) continue;
+        const float4 pi = a.pos_s[k], vi = a.vel_s[k], abi = a.abar[k];
+        const float2 xi = a.aux[k];
+        const float Pi = xi.x, qi = xi.y;
+        float gx = 0, gy = 0, gz = 0, hx = 0, hy = 0, hz = 0, Pb = 0, qb = 0, Ax = 0, Ay = 0, Az = 0;
+        for_each_neighbor(a, sc, k, pi, [&](uint32_t j) {
+            const float4 pj = a.pos_s[j], vj = a.vel_s[j], abj = a.abar[j];
+            const float2 xj = a.aux[j];
+            const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+            const float d2 = dist2(rx, ry, rz);
+            const float inv_r = d2 > 0.0f ? rsqrtf(d2) : 0.0f;
+            const float dd = d2 * inv_r;
+            const float hr = sc.h - dd;
+            const float tt = sc.h2 - d2;
+            const float phi = hr * hr * inv_r;
+            const float dphi_r = -tt * inv_r * inv_r * inv_r;
+            const float dax = abi.x - abj.x, day = abi.y - abj.y, daz = abi.z - abj.z;
+            const float dvx = vj.x - vi.x, dvy = vj.y - vi.y, dvz = vj.z - vi.z;
+            const float PP = sc.cP * (Pi + xj.x);
+            const float dar = dax * rx + day * ry + daz * rz;
+            const float dav = dax * dvx + day * dvy + daz * dvz;
+            const float cq = sc.cV * qi * xj.y;
+            const float coefr = PP * dphi_r * dar - cq * inv_r * dav;
+            gx += PP * phi * dax + coefr * rx;
+            gy += PP * phi * day + coefr * ry;
+            gz += PP * phi * daz + coefr * rz;
+            hx -= cq * hr * dax;
+            hy -= cq * hr * day;
+            hz -= cq * hr * daz;
+            Pb += sc.cP * phi * dar;
+            qb += sc.cV * xj.y * hr * dav;
+            Ax += hr * xj.y * dvx;
+            Ay += hr * xj.y * dvy;
+            Az += hr * xj.y * dvz;
+        });
+        // EOS derivatives (clamped Tait, exponent 7).
+        const float rho = a.rho[k];
+        const float x = rho * sc.inv_rho0;
+        const float x2 = x * x, x4 = x2 * x2, x6 = x4 * x2, x7 = x6 * x;
+        const float praw = sc.B * (x7 - 1.0f);
+        const float sgn = praw < 0.0f ? (sc.attractive ? sc.attr : 0.0f) : 1.0f;
+        const float p = praw * sgn;
+        const float dp_drho = sgn * sc.B * 7.0f * x6 * sc.inv_rho0;
+        const float dp_dB = sgn * (x7 - 1.0f);
+        a.rhobar[k] = Pb * (dp_drho * qi * qi - 2.0f * p * qi * qi * qi) - qb * qi * qi;
+        a.bpart[k] = Pb * qi * qi * dp_dB;
+        a.mpart[k] = sc.cP * qi * (abi.x * Ax + abi.y * Ay + abi.z * Az);
+        float4 xs = a.xs[k], vs = a.vs[k];
+        xs.x += gx; xs.y += gy; xs.z += gz;
+        vs.x += hx; vs.y += hy; vs.z += hz;
+        a.xs[k] = xs;
+        a.vs[k] = vs;
+    }
+}
+
+__global__ void adj_density_kernel(AdjArgs a) {
+    uint32_t t0, t1;
+    tile_span(a.ntiles, t0, t1);
+    for (uint32_t t = t0; t < t1; ++t) {
+        const uint2 ti = a.tiles[t];
+        const SceneConst& sc = a.scenes[ti.x];
+        const uint32_t k = ti.y + threadIdx.x;
+        if (k >= sc.p_end) continue;
+        const float4 pi = a.pos_s[k];
+        const float rbi = a.rhobar[k];
+        float gx = 0, gy = 0, gz = 0;
+        for_each_neighbor(a, sc, k, pi, [&](uint32_t j) {
+            const float4 pj = a.pos_s[j];
+            const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+            const float tt = sc.h2 - dist2(rx, ry, rz);
+            const float s = -6.0f * sc.cD * tt * tt * (rbi + a.rhobar[j]);
+            gx += s * rx;
+            gy += s * ry;
+            gz += s * rz;
+        });
+        float4 xs = a.xs[k];
+        xs.x += gx; xs.y += gy; xs.z += gz;
+        const uint32_t src = a.perm[k];
+        a.xbar[src] = xs;
+        a.vbar[src] = a.vs[k];
+    }
+}
+
+// Per-scene sum of a per-particle array in a fixed order (one block per
+// scene), added to a double accumulator: deterministic run to run.
+__global__ __launch_bounds__(256) void scene_reduce_kernel(const SceneConst* __restrict__ scenes,
+                                                           const float* __restrict__ part,
+                                                           double* __restrict__ acc) {
+    __shared__ double s[256];
+    const SceneConst& sc = scenes[blockIdx.x];
+    double v = 0.0;
+    for (uint32_t i = sc.p_begin + threadIdx.x; i < sc.p_end; i += 256) v += (double)part[i];
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) acc[blockIdx.x] += s[0];
+}
+
+__global__ void gather3_kernel(uint32_t n, const float* __restrict__ g, const uint32_t* __restrict__ id,
+                               float4* __restrict__ out) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const size_t i = id[k];
+    out[k] = make_float4(g[3 * i], g[3 * i + 1], g[3 * i + 2], 0.0f);
+}
+
+}  // namespace
+
+hipError_t launch_adj_integrate(hipStream_t st, int block, uint32_t grid, const AdjArgs& a) {
+    if (grid == 0) return hipSuccess;
+    hipLaunchKernelGGL(adj_integrate_kernel, dim3(grid), dim3(block), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_adj_force(hipStream_t st, int block, uint32_t grid, const AdjArgs& a) {
+    if (grid == 0) return hipSuccess;
+    hipLaunchKernelGGL(adj_force_kernel, dim3(grid), dim3(block), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_adj_density(hipStream_t st, int block, uint32_t grid, const AdjArgs& a) {
+    if (grid == 0) return hipSuccess;
+    hipLaunchKernelGGL(adj_density_kernel, dim3(grid), dim3(block), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_scene_reduce(hipStream_t st, uint32_t nscenes, const SceneConst* scenes, const float* part,
+                               double* acc) {
+    if (nscenes == 0) return hipSuccess;
+    hipLaunchKernelGGL(scene_reduce_kernel, dim3(nscenes), dim3(256), 0, st, scenes, part, acc);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather3(hipStream_t st, uint32_t n, const float* g, const uint32_t* id, float4* out) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(gather3_kernel, dim3((n + 255) / 256), dim3(256), 0, st, n, g, id, out);
+    return hipGetLastError();
+}
+
+}  // namespace sph

@@ -1,0 +1,816 @@
+// Host side of the engine: handle lifecycle, step enqueue, HIP graph replay,
+// per-pass profiling and the C ABI declared in include/sph_engine.h.
+#include "sph_engine.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+using namespace sph;
+
+sph_handle::~sph_handle() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (auto e : prof_events) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+namespace sph {
+
+int fail(sph_handle* h, int code, const std::string& msg) {
+    if (h) h->last_error = msg;
+    return code;
+}
+
+int hip_fail(sph_handle* h, hipError_t e, const char* what) {
+    if (e == hipSuccess) return SPH_OK;
+    const int code = (e == hipErrorOutOfMemory) ? SPH_E_OOM : SPH_E_HIP;
+    return fail(h, code, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+static bool finite3(const float* v) { return std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2]); }
+
+// Derived constants, formed exactly as the CPU restatement forms them:
+// normalisations in double from the float inputs, rounded once to float.
+int derive_scene(const sph_params& p, SceneConst& sc, std::string& err) {
+    if (!(p.particle_radius > 0.f) || !std::isfinite(p.particle_radius)) { err = "particle_radius must be > 0"; return SPH_E_INVALID; }
+    if (!(p.kernel_radius > 0.f) || !std::isfinite(p.kernel_radius)) { err = "kernel_radius must be > 0"; return SPH_E_INVALID; }
+    if (!(p.rest_density > 0.f) || !std::isfinite(p.rest_density)) { err = "rest_density must be > 0"; return SPH_E_INVALID; }
+    if (!(p.pressure_stiffness >= 0.f) || !std::isfinite(p.pressure_stiffness)) { err = "pressure_stiffness must be >= 0"; return SPH_E_INVALID; }
+    if (!(p.viscosity >= 0.f) || !std::isfinite(p.viscosity)) { err = "viscosity must be >= 0"; return SPH_E_INVALID; }
+    if (!(p.boundary_restitution >= 0.f) || !std::isfinite(p.boundary_restitution)) { err = "boundary_restitution must be >= 0"; return SPH_E_INVALID; }
+    if (!(p.dt > 0.f) || !std::isfinite(p.dt)) { err = "dt must be > 0"; return SPH_E_INVALID; }
+    if (!finite3(p.gravity) || !finite3(p.box_min) || !finite3(p.box_max)) { err = "gravity and box must be finite"; return SPH_E_INVALID; }
+    if (!std::isfinite(p.attractive_pressure_ratio)) { err = "attractive_pressure_ratio must be finite"; return SPH_E_INVALID; }
+    std::memset(&sc, 0, sizeof(sc));
+    const float r = p.particle_radius;
+    const float sp = 2.0f * r;
+    const float mass = p.rest_density * (sp * sp * sp);
+    sc.h = p.kernel_radius;
+    sc.h2 = sc.h * sc.h;
+    const double hd = (double)p.kernel_radius;
+    const double h3 = hd * hd * hd, h6 = h3 * h3, h9 = h6 * h3;
+    const double poly6 = 315.0 / (64.0 * M_PI * h9);
+    const double spiky = 45.0 / (M_PI * h6);
+    sc.cD = (float)((double)mass * poly6);
+    sc.cP = (float)((double)mass * spiky);
+    sc.cV = (float)((double)p.viscosity * (double)mass * spiky);
+    sc.inv_rho0 = (float)(1.0 / (double)p.rest_density);
+    sc.inv_cs = (float)(1.0 / (hd * (1.0 + 1.0 / 1024.0)));
+    sc.B = p.pressure_stiffness;
+    sc.attr = p.use_attractive_pressure ? p.attractive_pressure_ratio : 0.0f;
+    sc.attractive = p.use_attractive_pressure ? 1 : 0;
+    sc.e = p.boundary_restitution;
+    sc.dt = p.dt;
+    sc.mu = p.viscosity;
+    uint64_t cells = 1;
+    for (int k = 0; k < 3; ++k) {
+        sc.g[k] = p.gravity[k];
+        sc.bmin[k] = p.box_min[k];
+        const float ext = p.box_max[k] - p.box_min[k];
+        if (!(ext > 2.0f * r)) { err = "box must be wider than 2*particle_radius on every axis"; return SPH_E_INVALID; }
+        int G = (int)std::ceil(ext * sc.inv_cs);
+        if (G < 1) G = 1;
+        if (G > 1024) { err = "grid exceeds 1024 cells on an axis (box / kernel_radius too large)"; return SPH_E_INVALID; }
+        sc.G[k] = G;
+        sc.lo[k] = p.box_min[k] + r;
+        sc.hi[k] = p.box_max[k] - r;
+        cells *= (uint64_t)G;
+    }
+    if (cells > (1ull << 30)) { err = "too many grid cells"; return SPH_E_INVALID; }
+    return SPH_OK;
+}
+
+void invalidate_graph(sph_handle* h) {
+    if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+    if (h->graph) (void)hipGraphDestroy(h->graph);
+    h->gexec = nullptr;
+    h->graph = nullptr;
+}
+
+uint2* tiles_for(sph_handle* h, int block, uint32_t& ntiles) {
+    auto it = h->tiles.find(block);
+    if (it != h->tiles.end() && it->second.p) {
+        ntiles = h->ntiles[block];
+        return it->second.p;
+    }
+    std::vector<uint2> t;
+    for (uint32_t s = 0; s < (uint32_t)h->sc_h.size(); ++s)
+        for (uint32_t b = h->sc_h[s].p_begin; b < h->sc_h[s].p_end; b += (uint32_t)block)
+            t.push_back(make_uint2(s, b));
+    DevBuf<uint2>& d = h->tiles[block];
+    if (d.ensure(std::max<size_t>(t.size(), 1)) != hipSuccess) { ntiles = 0; return nullptr; }
+    if (!t.empty()) (void)hipMemcpy(d.p, t.data(), t.size() * sizeof(uint2), hipMemcpyHostToDevice);
+    h->ntiles[block] = (uint32_t)t.size();
+    ntiles = (uint32_t)t.size();
+    return d.p;
+}
+
+uint32_t grid_for(sph_handle* h, int block, uint32_t ntiles) {
+    if (!h->persistent) return ntiles;
+    const uint32_t resident = (uint32_t)h->cu_count * (uint32_t)std::max(1, 2048 / block);
+    return std::min(ntiles, resident);
+}
+
+StepArgs step_args(sph_handle* h, StepBufs& w, int block) {
+    StepArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.pos_s = w.pos_s.p;
+    a.vel_s = w.vel_s.p;
+    a.pos_o = h->pos.p;
+    a.vel_o = h->vel.p;
+    a.id_s = w.id_s.p;
+    a.id_o = h->id.p;
+    a.key0 = h->key0.p;
+    a.cells = w.cells.p;
+    a.rho = w.rho.p;
+    a.aux = w.aux.p;
+    a.mask = w.mask.p;
+    a.acc = (h->keep_acc || w.acc.p) ? w.acc.p : nullptr;
+    a.scenes = h->sc_d.p;
+    a.tiles = tiles_for(h, block, a.ntiles);
+    a.n = h->n;
+    return a;
+}
+
+int alloc_step_bufs(sph_handle* h, StepBufs& w) {
+    const size_t n = h->n;
+    hipError_t e = hipSuccess;
+    auto chk = [&](hipError_t x) { if (x != hipSuccess && e == hipSuccess) e = x; };
+    chk(w.pos_s.ensure(n));
+    chk(w.vel_s.ensure(n));
+    chk(w.id_s.ensure(n));
+    for (int i = 0; i < 2; ++i) { chk(w.kbuf[i].ensure(n)); chk(w.vbuf[i].ensure(n)); }
+    const bool fresh_cells = w.cells.count < h->ncells || !w.cells.p;
+    chk(w.cells.ensure(h->ncells));
+    chk(w.rho.ensure(n));
+    chk(w.aux.ensure(n));
+    chk(w.mask.ensure((size_t)kRows * n));
+    if (h->keep_acc) chk(w.acc.ensure(n));
+    if (e != hipSuccess) return hip_fail(h, e, "device allocation");
+    if (fresh_cells || !w.cells_clean) {
+        chk(launch_fill_cells(h->stream, w.cells.p, h->ncells));
+        w.cells_clean = true;
+        w.prev_valid = false;
+    }
+    return hip_fail(h, e, "cell table clear");
+}
+
+static int ensure_buffers(sph_handle* h) {
+    const size_t n = h->n;
+    hipError_t e = hipSuccess;
+    auto chk = [&](hipError_t x) { if (x != hipSuccess && e == hipSuccess) e = x; };
+    chk(h->pos.ensure(n));
+    chk(h->vel.ensure(n));
+    chk(h->id.ensure(n));
+    chk(h->key0.ensure(n));
+    chk(h->hist.ensure(kMaxPasses * 256));
+    chk(h->status.ensure((size_t)kMaxPasses * sort_tiles((uint32_t)n, 4) * 256 + 256));
+    chk(h->tickets.ensure(kMaxPasses));
+    chk(h->err.ensure(1));
+    chk(h->staging.ensure(3 * n + 1));
+    chk(h->sc_d.ensure(h->sc_h.size()));
+    if (e != hipSuccess) return hip_fail(h, e, "device allocation");
+    chk(launch_fill_u32(h->stream, h->hist.p, 0u, kMaxPasses * 256));
+    chk(launch_fill_u32(h->stream, h->err.p, 0u, 1));
+    chk(hipMemcpyAsync(h->sc_d.p, h->sc_h.data(), h->sc_h.size() * sizeof(SceneConst), hipMemcpyHostToDevice, h->stream));
+    if (e != hipSuccess) return hip_fail(h, e, "buffer init");
+    return alloc_step_bufs(h, h->work);
+}
+
+static void prof_record(sph_handle* h, int slot) {
+    const size_t idx = h->prof_used + slot;
+    while (h->prof_events.size() <= idx) {
+        hipEvent_t ev;
+        if (hipEventCreate(&ev) != hipSuccess) return;
+        h->prof_events.push_back(ev);
+    }
+    (void)hipEventRecord(h->prof_events[idx], h->stream);
+}
+
+int enqueue_step(sph_handle* h, StepBufs& w, bool record_prof) {
+    hipStream_t st = h->stream;
+    if (record_prof) prof_record(h, 0);
+    if (!h->keys_valid) {
+        uint32_t nt = 0;
+        const uint2* tl = tiles_for(h, 256, nt);
+        hipError_t e = launch_hash(st, 256, nt, h->pos.p, h->key0.p, h->sc_d.p, tl, nt);
+        if (e != hipSuccess) return hip_fail(h, e, "hash launch");
+        h->keys_valid = true;
+    }
+    if (record_prof) prof_record(h, 1);
+    SortPlan sp;
+    sp.n = h->n;
+    sp.passes = h->passes;
+    sp.items = h->tune.sort_items;
+    sp.keys_in = h->key0.p;
+    for (int i = 0; i < 2; ++i) { sp.kbuf[i] = w.kbuf[i].p; sp.vbuf[i] = w.vbuf[i].p; }
+    sp.hist = h->hist.p;
+    sp.status = h->status.p;
+    sp.tickets = h->tickets.p;
+    sp.err = h->err.p;
+    const int fin = sort_final(h->passes);
+    sp.prev_sorted = w.kbuf[fin].p;
+    sp.prev_n = w.prev_valid ? h->n : 0u;
+    sp.cells = w.cells.p;
+    hipError_t e = launch_radix_sort(st, sp);
+    if (e != hipSuccess) return hip_fail(h, e, "sort launch");
+    if (record_prof) prof_record(h, 2);
+    e = launch_reorder(st, h->tune.reorder_block, h->n, w.vbuf[fin].p, w.kbuf[fin].p, h->pos.p, h->vel.p,
+                       h->id.p, w.pos_s.p, w.vel_s.p, w.id_s.p, w.cells.p, h->ncells, h->hist.p, h->err.p);
+    if (e != hipSuccess) return hip_fail(h, e, "reorder launch");
+    w.prev_valid = true;
+    w.cells_clean = false;
+    if (record_prof) prof_record(h, 3);
+    {
+        StepArgs a = step_args(h, w, h->tune.density_block);
+        e = launch_density(st, h->tune.density_block, grid_for(h, h->tune.density_block, a.ntiles), a);
+        if (e != hipSuccess) return hip_fail(h, e, "density launch");
+    }
+    if (record_prof) prof_record(h, 4);
+    {
+        StepArgs a = step_args(h, w, h->tune.force_block);
+        e = launch_force(st, h->tune.force_block, grid_for(h, h->tune.force_block, a.ntiles), a);
+        if (e != hipSuccess) return hip_fail(h, e, "force launch");
+    }
+    if (record_prof) {
+        prof_record(h, 5);
+        h->prof_used += 6;
+    }
+    h->has_step = true;
+    return SPH_OK;
+}
+
+int check_device_error(sph_handle* h) {
+    uint32_t flag = 0;
+    hipError_t e = hipMemcpy(&flag, h->err.p, sizeof(uint32_t), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(h, e, "error flag read");
+    if (flag) {
+        std::string what;
+        if (flag & 1u) what += " sort look-back wait exceeded its bound;";
+        if (flag & 2u) what += " sort scatter index out of range;";
+        if (flag & 4u) what += " reorder index out of range;";
+        return fail(h, SPH_E_DEVICE, "device error flag set:" + what);
+    }
+    return SPH_OK;
+}
+
+int sync(sph_handle* h) {
+    hipError_t e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return hip_fail(h, e, "stream synchronize");
+    return check_device_error(h);
+}
+
+}  // namespace sph
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+
+static int configure(sph_handle* h, const std::vector<sph_params>& ps, const std::vector<uint32_t>& ns) {
+    std::vector<SceneConst> sc(ps.size());
+    uint64_t cell_off = 0, p_off = 0;
+    std::vector<uint32_t> off(ps.size() + 1, 0);
+    for (size_t s = 0; s < ps.size(); ++s) {
+        std::string err;
+        const int rc = derive_scene(ps[s], sc[s], err);
+        if (rc != SPH_OK) return fail(h, rc, ps.size() > 1 ? "scene " + std::to_string(s) + ": " + err : err);
+        if (ps[s].dt != ps[0].dt) return fail(h, SPH_E_INVALID, "all scenes of a batch must share dt");
+        sc[s].cell_offset = (uint32_t)cell_off;
+        cell_off += (uint64_t)sc[s].G[0] * sc[s].G[1] * sc[s].G[2];
+        sc[s].p_begin = (uint32_t)p_off;
+        p_off += ns[s];
+        sc[s].p_end = (uint32_t)p_off;
+        off[s + 1] = (uint32_t)p_off;
+    }
+    if (cell_off > (1ull << 30)) return fail(h, SPH_E_INVALID, "too many grid cells in total");
+    if (p_off >= (1ull << 30)) return fail(h, SPH_E_INVALID, "too many particles (limit 2^30)");
+    h->params = ps;
+    h->scene_n = ns;
+    h->scene_off = off;
+    h->sc_h = sc;
+    h->n = (uint32_t)p_off;
+    h->ncells = (uint32_t)cell_off;
+    h->passes = sort_key_passes(h->ncells);
+    h->tiles.clear();
+    h->ntiles.clear();
+    invalidate_graph(h);
+    return SPH_OK;
+}
+
+static int new_handle(sph_handle** out) {
+    if (!out) return SPH_E_INVALID;
+    *out = nullptr;
+    sph_handle* h = new (std::nothrow) sph_handle();
+    if (!h) return SPH_E_OOM;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) {
+        hipDeviceProp_t prop;
+        e = hipGetDeviceProperties(&prop, dev);
+        if (e == hipSuccess) {
+            h->cu_count = prop.multiProcessorCount;
+            h->arch = prop.gcnArchName;
+        }
+    }
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete h;
+        return SPH_E_HIP;
+    }
+    *out = h;
+    return SPH_OK;
+}
+
+extern "C" {
+
+int sph_params_default(sph_params* p) {
+    if (!p) return SPH_E_INVALID;
+    std::memset(p, 0, sizeof(*p));
+    p->rest_density = 1000.0f;
+    p->pressure_stiffness = 50000.0f;
+    p->viscosity = 0.05f;
+    p->boundary_restitution = 0.1f;
+    p->use_attractive_pressure = 0;
+    p->attractive_pressure_ratio = 0.1f;
+    p->gravity[0] = 0.0f; p->gravity[1] = -9.81f; p->gravity[2] = 0.0f;
+    p->dt = 1e-4f;
+    for (int k = 0; k < 3; ++k) { p->box_min[k] = -1.0f; p->box_max[k] = 1.0f; }
+    p->stop_time = 5.0f;
+    p->num_threads = 0;
+    return sph_params_set_resolution(p, 24);
+}
+
+int sph_params_set_resolution(sph_params* p, int res) {
+    if (!p || res <= 0) return SPH_E_INVALID;
+    p->kernel_radius = 2.0f / (float)res;
+    p->particle_radius = p->kernel_radius / 4.0f;
+    return SPH_OK;
+}
+
+int sph_create(const sph_params* p, sph_handle** out) {
+    if (!p || !out) return SPH_E_INVALID;
+    sph_handle* h = nullptr;
+    int rc = new_handle(&h);
+    if (rc != SPH_OK) return rc;
+    SceneConst tmp;
+    std::string err;
+    rc = derive_scene(*p, tmp, err);
+    if (rc != SPH_OK) {
+        delete h;
+        return rc;
+    }
+    h->params.assign(1, *p);
+    h->scene_n.assign(1, 0);
+    *out = h;
+    return SPH_OK;
+}
+
+void sph_destroy(sph_handle* h) {
+    if (!h) return;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    delete h;
+}
+
+int sph_set_params(sph_handle* h, const sph_params* p) {
+    if (!h || !p) return SPH_E_INVALID;
+    if (h->batched) return fail(h, SPH_E_INVALID, "use sph_set_scene_params on a batch handle");
+    SceneConst tmp;
+    std::string err;
+    int rc = derive_scene(*p, tmp, err);
+    if (rc != SPH_OK) return fail(h, rc, err);
+    h->params[0] = *p;
+    if (h->has_particles) {
+        rc = configure(h, h->params, h->scene_n);
+        if (rc != SPH_OK) return rc;
+        h->ready = false;
+        h->keys_valid = false;
+    }
+    return SPH_OK;
+}
+
+int sph_get_params(const sph_handle* h, sph_params* out) {
+    if (!h || !out) return SPH_E_INVALID;
+    *out = h->params[0];
+    return SPH_OK;
+}
+
+static int upload_state(sph_handle* h, const float* xyz, const float* vel, const uint32_t* ids) {
+    const size_t n = h->n;
+    int rc = ensure_buffers(h);
+    if (rc != SPH_OK) return rc;
+    if (n == 0) return SPH_OK;
+    // Stage through the handle's 3n scratch: positions, then velocities.
+    hipError_t e = hipMemcpyAsync(h->staging.p, xyz, 3 * n * sizeof(float), hipMemcpyHostToDevice, h->stream);
+    if (e == hipSuccess) e = launch_pack3(h->stream, (uint32_t)n, h->staging.p, h->pos.p);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e == hipSuccess && vel) {
+        e = hipMemcpyAsync(h->staging.p, vel, 3 * n * sizeof(float), hipMemcpyHostToDevice, h->stream);
+        if (e == hipSuccess) e = launch_pack3(h->stream, (uint32_t)n, h->staging.p, h->vel.p);
+    } else if (e == hipSuccess) {
+        e = launch_pack3(h->stream, (uint32_t)n, nullptr, h->vel.p);
+    }
+    if (e == hipSuccess) {
+        if (ids) {
+            e = hipMemcpyAsync(h->id.p, ids, n * sizeof(uint32_t), hipMemcpyHostToDevice, h->stream);
+        } else {
+            std::vector<uint32_t> iota(n);
+            for (size_t i = 0; i < n; ++i) iota[i] = (uint32_t)i;
+            e = hipMemcpyAsync(h->id.p, iota.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, h->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+        }
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return hip_fail(h, e, "state upload");
+    h->keys_valid = false;
+    h->work.prev_valid = false;
+    if (!h->work.cells_clean) {
+        e = launch_fill_cells(h->stream, h->work.cells.p, h->ncells);
+        if (e != hipSuccess) return hip_fail(h, e, "cell table clear");
+        h->work.cells_clean = true;
+    }
+    invalidate_graph(h);
+    return SPH_OK;
+}
+
+int sph_set_particles(sph_handle* h, const float* xyz, const float* vel, uint32_t n) {
+    if (!h) return SPH_E_INVALID;
+    if (n > 0 && !xyz) return fail(h, SPH_E_INVALID, "xyz is NULL");
+    if (h->batched) {
+        if (n != h->n) return fail(h, SPH_E_INVALID, "a batch takes exactly the sum of its scene counts");
+    } else {
+        int rc = configure(h, h->params, std::vector<uint32_t>(1, n));
+        if (rc != SPH_OK) return rc;
+    }
+    h->has_particles = true;
+    h->ready = false;
+    h->has_step = false;
+    return upload_state(h, xyz, vel, nullptr);
+}
+
+int sph_make_ready(sph_handle* h) {
+    if (!h) return SPH_E_INVALID;
+    if (!h->has_particles) return fail(h, SPH_E_NOT_READY, "no particles set");
+    if (h->n == 0) return fail(h, SPH_E_NOT_READY, "zero particles");
+    int rc = ensure_buffers(h);
+    if (rc != SPH_OK) return rc;
+    h->ready = true;
+    return SPH_OK;
+}
+
+static int step_once(sph_handle* h) {
+    if (!h->ready) return fail(h, SPH_E_NOT_READY, "call sph_make_ready first");
+    return enqueue_step(h, h->work, h->prof);
+}
+
+static int capture_graph(sph_handle* h) {
+    invalidate_graph(h);
+    hipError_t e = hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return hip_fail(h, e, "begin capture");
+    const int rc = enqueue_step(h, h->work, false);
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(h->stream, &g);
+    if (rc != SPH_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return hip_fail(h, e, "end capture");
+    h->graph = g;
+    e = hipGraphInstantiate(&h->gexec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        invalidate_graph(h);
+        return hip_fail(h, e, "graph instantiate");
+    }
+    return SPH_OK;
+}
+
+int sph_advance_frame(sph_handle* h, float* dt_out) {
+    if (!h) return SPH_E_INVALID;
+    int rc = sph_advance_steps(h, 1);
+    if (rc != SPH_OK) return rc;
+    if (dt_out) *dt_out = h->params[0].dt;
+    return SPH_OK;
+}
+
+int sph_advance_steps(sph_handle* h, uint32_t k) {
+    if (!h) return SPH_E_INVALID;
+    if (!h->ready) return fail(h, SPH_E_NOT_READY, "call sph_make_ready first");
+    for (uint32_t i = 0; i < k; ++i) {
+        // The graph holds a steady-state step: keys from the last integrate,
+        // a cell table to clear sparsely.  Earlier steps run eagerly.
+        const bool steady = h->keys_valid && h->work.prev_valid && !h->keep_acc;
+        if (h->use_graph && steady && !h->prof) {
+            if (!h->gexec) {
+                int rc = capture_graph(h);
+                if (rc != SPH_OK) return rc;
+            }
+            hipError_t e = hipGraphLaunch(h->gexec, h->stream);
+            if (e != hipSuccess) return hip_fail(h, e, "graph launch");
+        } else {
+            int rc = step_once(h);
+            if (rc != SPH_OK) return rc;
+        }
+    }
+    return SPH_OK;
+}
+
+int sph_run_frame(sph_handle* h, float* frame_time_out, uint32_t* steps_out) {
+    if (!h) return SPH_E_INVALID;
+    float frame_time = 0.0f;
+    uint32_t steps = 0;
+    while (frame_time < 0.0333333333) {
+        float dt = 0.0f;
+        int rc = sph_advance_frame(h, &dt);
+        if (rc != SPH_OK) return rc;
+        frame_time += dt;
+        ++steps;
+    }
+    if (frame_time_out) *frame_time_out = frame_time;
+    if (steps_out) *steps_out = steps;
+    return SPH_OK;
+}
+
+uint32_t sph_num_particles(const sph_handle* h) { return h ? h->n : 0u; }
+
+static int fetch3(sph_handle* h, const float4* src, float* out) {
+    if (!h || !out) return SPH_E_INVALID;
+    if (!h->has_particles) return fail(h, SPH_E_NOT_READY, "no particles set");
+    if (h->n == 0) return SPH_OK;
+    hipError_t e = launch_unpermute3(h->stream, h->n, src, h->id.p, h->staging.p);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, h->staging.p, 3 * (size_t)h->n * sizeof(float), hipMemcpyDeviceToHost, h->stream);
+    if (e != hipSuccess) return hip_fail(h, e, "particle download");
+    return sync(h);
+}
+
+int sph_get_particles(sph_handle* h, float* xyz_out) { return fetch3(h, h ? h->pos.p : nullptr, xyz_out); }
+int sph_get_velocity(sph_handle* h, float* vel_out) { return fetch3(h, h ? h->vel.p : nullptr, vel_out); }
+
+int sph_get_density(sph_handle* h, float* rho_out) {
+    if (!h || !rho_out) return SPH_E_INVALID;
+    if (!h->has_step) return fail(h, SPH_E_NOT_READY, "no step has run");
+    // Density is in the last step's sorted order, which is the state order.
+    hipError_t e = launch_unpermute1(h->stream, h->n, h->work.rho.p, h->id.p, h->staging.p);
+    if (e == hipSuccess) e = hipMemcpyAsync(rho_out, h->staging.p, (size_t)h->n * sizeof(float), hipMemcpyDeviceToHost, h->stream);
+    if (e != hipSuccess) return hip_fail(h, e, "density download");
+    return sync(h);
+}
+
+int sph_synchronize(sph_handle* h) {
+    if (!h) return SPH_E_INVALID;
+    return sync(h);
+}
+
+const char* sph_last_error(const sph_handle* h) { return h ? h->last_error.c_str() : "null handle"; }
+
+int sph_set_use_graph(sph_handle* h, int on) {
+    if (!h) return SPH_E_INVALID;
+    h->use_graph = on != 0;
+    if (!h->use_graph) invalidate_graph(h);
+    return SPH_OK;
+}
+
+// ---- batched ----
+
+int sph_batch_create(const sph_scene_desc* scenes, uint32_t nscenes, sph_handle** out) {
+    if (!scenes || nscenes == 0 || !out) return SPH_E_INVALID;
+    if (nscenes > 65535) return SPH_E_INVALID;
+    sph_handle* h = nullptr;
+    int rc = new_handle(&h);
+    if (rc != SPH_OK) return rc;
+    std::vector<sph_params> ps(nscenes);
+    std::vector<uint32_t> ns(nscenes);
+    for (uint32_t s = 0; s < nscenes; ++s) { ps[s] = scenes[s].params; ns[s] = scenes[s].num_particles; }
+    h->batched = true;
+    h->persistent = true;
+    rc = configure(h, ps, ns);
+    if (rc != SPH_OK) {
+        delete h;
+        return rc;
+    }
+    rc = ensure_buffers(h);
+    if (rc != SPH_OK) {
+        delete h;
+        return rc;
+    }
+    *out = h;
+    return SPH_OK;
+}
+
+uint32_t sph_num_scenes(const sph_handle* h) { return h ? (uint32_t)h->sc_h.size() : 0u; }
+
+int sph_scene_offsets(const sph_handle* h, uint32_t* offsets) {
+    if (!h || !offsets) return SPH_E_INVALID;
+    std::copy(h->scene_off.begin(), h->scene_off.end(), offsets);
+    return SPH_OK;
+}
+
+int sph_set_scene_params(sph_handle* h, uint32_t scene, const sph_params* p) {
+    if (!h || !p) return SPH_E_INVALID;
+    if (scene >= h->sc_h.size()) return fail(h, SPH_E_INVALID, "scene index out of range");
+    const sph_params& o = h->params[scene];
+    const bool same_geom = p->particle_radius == o.particle_radius && p->kernel_radius == o.kernel_radius &&
+                           p->dt == o.dt && p->rest_density == o.rest_density &&
+                           std::memcmp(p->box_min, o.box_min, sizeof(o.box_min)) == 0 &&
+                           std::memcmp(p->box_max, o.box_max, sizeof(o.box_max)) == 0;
+    if (!same_geom) return fail(h, SPH_E_INVALID, "box, radii, rest density and dt are fixed at create time");
+    SceneConst sc;
+    std::string err;
+    int rc = derive_scene(*p, sc, err);
+    if (rc != SPH_OK) return fail(h, rc, err);
+    sc.cell_offset = h->sc_h[scene].cell_offset;
+    sc.p_begin = h->sc_h[scene].p_begin;
+    sc.p_end = h->sc_h[scene].p_end;
+    h->params[scene] = *p;
+    h->sc_h[scene] = sc;
+    hipError_t e = hipMemcpyAsync(h->sc_d.p + scene, &h->sc_h[scene], sizeof(SceneConst), hipMemcpyHostToDevice, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    return hip_fail(h, e, "scene table upload");
+}
+
+// ---- measurement ----
+
+int sph_profile_enable(sph_handle* h, int on) {
+    if (!h) return SPH_E_INVALID;
+    h->prof = on != 0;
+    return SPH_OK;
+}
+
+int sph_profile_read(sph_handle* h, double* ms, uint32_t* steps) {
+    if (!h) return SPH_E_INVALID;
+    hipError_t e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return hip_fail(h, e, "stream synchronize");
+    for (size_t b = 0; b + 6 <= h->prof_used; b += 6) {
+        for (int p = 0; p < SPH_NUM_PASSES; ++p) {
+            float t = 0.f;
+            if (hipEventElapsedTime(&t, h->prof_events[b + p], h->prof_events[b + p + 1]) == hipSuccess)
+                h->prof_ms[p] += t;
+        }
+        h->prof_steps++;
+    }
+    h->prof_used = 0;
+    if (ms) for (int p = 0; p < SPH_NUM_PASSES; ++p) ms[p] = h->prof_ms[p];
+    if (steps) *steps = h->prof_steps;
+    for (int p = 0; p < SPH_NUM_PASSES; ++p) h->prof_ms[p] = 0;
+    h->prof_steps = 0;
+    return SPH_OK;
+}
+
+int sph_get_tuning(const sph_handle* h, int* cfg) {
+    if (!h || !cfg) return SPH_E_INVALID;
+    cfg[0] = h->tune.density_block;
+    cfg[1] = h->tune.force_block;
+    cfg[2] = h->tune.reorder_block;
+    cfg[3] = h->tune.sort_items;
+    return SPH_OK;
+}
+
+static bool valid_block(int b) { return b == 64 || b == 128 || b == 256 || b == 512; }
+
+int sph_set_tuning(sph_handle* h, const int* cfg) {
+    if (!h || !cfg) return SPH_E_INVALID;
+    if (!valid_block(cfg[0]) || !valid_block(cfg[1]) || !valid_block(cfg[2]))
+        return fail(h, SPH_E_INVALID, "block sizes must be 64, 128, 256 or 512");
+    if (cfg[3] != 4 && cfg[3] != 8 && cfg[3] != 12 && cfg[3] != 16)
+        return fail(h, SPH_E_INVALID, "sort items must be 4, 8, 12 or 16");
+    h->tune.density_block = cfg[0];
+    h->tune.force_block = cfg[1];
+    h->tune.reorder_block = cfg[2];
+    h->tune.sort_items = cfg[3];
+    invalidate_graph(h);
+    return SPH_OK;
+}
+
+// ---- test-only views ----
+
+int sph_debug_constants(const sph_handle* h, uint32_t scene, float* out14, int* G3) {
+    if (!h || !out14 || !G3) return SPH_E_INVALID;
+    SceneConst sc;
+    if (scene < h->sc_h.size()) {
+        sc = h->sc_h[scene];
+    } else if (scene == 0) {
+        std::string err;
+        if (derive_scene(h->params[0], sc, err) != SPH_OK) return SPH_E_INVALID;
+    } else {
+        return SPH_E_INVALID;
+    }
+    const sph_params& p = h->params[scene < h->params.size() ? scene : 0];
+    const float sp = 2.0f * p.particle_radius;
+    const float mass = p.rest_density * (sp * sp * sp);
+    const float v[14] = {sc.h, sc.h2, sc.inv_cs, mass, sc.cD, sc.cP, sc.cV,
+                         sc.inv_rho0, sc.lo[0], sc.lo[1], sc.lo[2], sc.hi[0], sc.hi[1], sc.hi[2]};
+    std::copy(v, v + 14, out14);
+    for (int k = 0; k < 3; ++k) G3[k] = sc.G[k];
+    return SPH_OK;
+}
+
+int sph_debug_get_state(sph_handle* h, float* xyz, float* vel, uint32_t* ids) {
+    if (!h) return SPH_E_INVALID;
+    if (!h->has_particles) return fail(h, SPH_E_NOT_READY, "no particles set");
+    const size_t n = h->n;
+    std::vector<float4> tmp(n);
+    hipError_t e = hipStreamSynchronize(h->stream);
+    if (e == hipSuccess && xyz) {
+        e = hipMemcpy(tmp.data(), h->pos.p, n * sizeof(float4), hipMemcpyDeviceToHost);
+        for (size_t i = 0; i < n && e == hipSuccess; ++i) { xyz[3 * i] = tmp[i].x; xyz[3 * i + 1] = tmp[i].y; xyz[3 * i + 2] = tmp[i].z; }
+    }
+    if (e == hipSuccess && vel) {
+        e = hipMemcpy(tmp.data(), h->vel.p, n * sizeof(float4), hipMemcpyDeviceToHost);
+        for (size_t i = 0; i < n && e == hipSuccess; ++i) { vel[3 * i] = tmp[i].x; vel[3 * i + 1] = tmp[i].y; vel[3 * i + 2] = tmp[i].z; }
+    }
+    if (e == hipSuccess && ids) e = hipMemcpy(ids, h->id.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(h, e, "state download");
+    return check_device_error(h);
+}
+
+int sph_debug_set_state(sph_handle* h, const float* xyz, const float* vel, const uint32_t* ids) {
+    if (!h || !xyz) return SPH_E_INVALID;
+    if (ids) {  // ids index the original-order outputs: they must be a permutation of 0..n-1
+        std::vector<uint8_t> seen(h->n, 0);
+        for (uint32_t k = 0; k < h->n; ++k) {
+            if (ids[k] >= h->n || seen[ids[k]]) return fail(h, SPH_E_INVALID, "ids must be a permutation of 0..n-1");
+            seen[ids[k]] = 1;
+        }
+    }
+    if (!h->has_particles) return fail(h, SPH_E_NOT_READY, "no particles set");
+    return upload_state(h, xyz, vel, ids);
+}
+
+int sph_debug_keep_acc(sph_handle* h, int on) {
+    if (!h) return SPH_E_INVALID;
+    h->keep_acc = on != 0;
+    invalidate_graph(h);
+    if (h->keep_acc && h->n) {
+        if (h->work.acc.ensure(h->n) != hipSuccess) return fail(h, SPH_E_OOM, "acc buffer");
+    }
+    return SPH_OK;
+}
+
+int sph_debug_step_outputs(sph_handle* h, uint32_t* keys_in, uint32_t* perm, uint32_t* sorted_keys,
+                           float* rho, float* acc3) {
+    if (!h) return SPH_E_INVALID;
+    if (!h->has_step) return fail(h, SPH_E_NOT_READY, "no step has run");
+    int rc = sync(h);
+    if (rc != SPH_OK) return rc;
+    const size_t n = h->n;
+    const int fin = sort_final(h->passes);
+    hipError_t e = hipSuccess;
+    (void)keys_in;
+    if (perm) e = hipMemcpy(perm, h->work.vbuf[fin].p, n * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && sorted_keys) e = hipMemcpy(sorted_keys, h->work.kbuf[fin].p, n * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && rho) e = hipMemcpy(rho, h->work.rho.p, n * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && acc3) {
+        if (!h->work.acc.p) return fail(h, SPH_E_NOT_READY, "enable sph_debug_keep_acc before stepping");
+        std::vector<float4> tmp(n);
+        e = hipMemcpy(tmp.data(), h->work.acc.p, n * sizeof(float4), hipMemcpyDeviceToHost);
+        for (size_t i = 0; i < n; ++i) { acc3[3 * i] = tmp[i].x; acc3[3 * i + 1] = tmp[i].y; acc3[3 * i + 2] = tmp[i].z; }
+    }
+    if (e == hipSuccess && keys_in) {
+        // Unsorted keys of the last step's input: keys_in[perm[k]] = sorted_keys[k].
+        std::vector<uint32_t> pm(n), sk(n);
+        e = hipMemcpy(pm.data(), h->work.vbuf[fin].p, n * 4, hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(sk.data(), h->work.kbuf[fin].p, n * 4, hipMemcpyDeviceToHost);
+        for (size_t k = 0; k < n && e == hipSuccess; ++k) keys_in[pm[k]] = sk[k];
+    }
+    return hip_fail(h, e, "debug download");
+}
+
+uint32_t sph_debug_num_cells(const sph_handle* h) { return h ? h->ncells : 0u; }
+
+int sph_debug_cells(sph_handle* h, uint32_t* start, uint32_t* end) {
+    if (!h || !start || !end) return SPH_E_INVALID;
+    int rc = sync(h);
+    if (rc != SPH_OK) return rc;
+    std::vector<uint2> c(h->ncells);
+    hipError_t e = hipMemcpy(c.data(), h->work.cells.p, c.size() * sizeof(uint2), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(h, e, "cell download");
+    for (size_t i = 0; i < c.size(); ++i) { start[i] = c[i].x; end[i] = c[i].y; }
+    return SPH_OK;
+}
+
+int sph_debug_neighbors(sph_handle* h, uint32_t* offsets, uint32_t* list, uint64_t capacity, uint64_t* count) {
+    if (!h || !offsets) return SPH_E_INVALID;
+    if (!h->has_step) return fail(h, SPH_E_NOT_READY, "no step has run");
+    const size_t n = h->n;
+    DevBuf<uint32_t> cnt, off, lst;
+    if (cnt.ensure(n + 1) != hipSuccess || off.ensure(n + 1) != hipSuccess) return fail(h, SPH_E_OOM, "neighbour buffers");
+    StepArgs a = step_args(h, h->work, 256);
+    hipError_t e = launch_neighbors(h->stream, 256, a.ntiles, a, nullptr, cnt.p, 0);
+    std::vector<uint32_t> c(n);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e == hipSuccess) e = hipMemcpy(c.data(), cnt.p, n * 4, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(h, e, "neighbour count");
+    uint64_t total = 0;
+    offsets[0] = 0;
+    for (size_t i = 0; i < n; ++i) { total += c[i]; offsets[i + 1] = (uint32_t)total; }
+    if (count) *count = total;
+    if (!list) return SPH_OK;
+    if (total > capacity) return fail(h, SPH_E_CAPACITY, "neighbour list capacity too small");
+    if (lst.ensure(std::max<uint64_t>(total, 1)) != hipSuccess) return fail(h, SPH_E_OOM, "neighbour list");
+    e = hipMemcpy(off.p, offsets, (n + 1) * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_neighbors(h->stream, 256, a.ntiles, a, off.p, lst.p, 1);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e == hipSuccess && total) e = hipMemcpy(list, lst.p, total * 4, hipMemcpyDeviceToHost);
+    return hip_fail(h, e, "neighbour fill");
+}
+
+}  // extern "C"

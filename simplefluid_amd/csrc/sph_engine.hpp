@@ -1,0 +1,142 @@
+// Engine handle shared by the C-ABI translation units.
+#pragma once
+
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "sph_engine.h"
+#include "sph_internal.hpp"
+
+namespace sph {
+
+// Device buffer owned by the handle.
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t count = 0;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        count = 0;
+    }
+    hipError_t ensure(size_t n) {
+        if (n <= count && p) return hipSuccess;
+        release();
+        if (n == 0) return hipSuccess;
+        hipError_t e = hipMalloc((void**)&p, n * sizeof(T));
+        if (e != hipSuccess) { p = nullptr; return e; }
+        count = n;
+        return hipSuccess;
+    }
+};
+
+// One step's intermediate buffers; the rollout keeps one per step of a
+// checkpoint segment (its tape), the plain step uses slot 0.
+struct StepBufs {
+    DevBuf<float4> pos_s, vel_s;   // sorted input state
+    DevBuf<uint32_t> id_s;
+    DevBuf<uint32_t> kbuf[2], vbuf[2];
+    DevBuf<uint2> cells;
+    DevBuf<float> rho;
+    DevBuf<float2> aux;
+    DevBuf<uint32_t> mask;
+    DevBuf<float4> acc;
+    bool cells_clean = false;      // cells all-empty (fresh)
+    bool prev_valid = false;       // kbuf[final] holds the keys that wrote cells
+};
+
+struct Tuning {
+    int density_block = 256;
+    int force_block = 256;
+    int reorder_block = 256;
+    int sort_items = 8;
+};
+
+// Differentiable rollout: a checkpoint of the state every `every` steps and
+// a tape of one segment's step intermediates, rebuilt segment by segment in
+// the reverse pass.
+struct RolloutState {
+    uint32_t steps = 0;
+    uint32_t every = 1;
+    uint32_t nseg = 0;
+    bool valid = false;
+    std::vector<std::unique_ptr<StepBufs>> tape;   // `every` slots
+    DevBuf<float4> ck_pos, ck_vel;                 // (nseg + 1) x n; slot nseg = final state
+    DevBuf<uint32_t> ck_id;
+    DevBuf<float4> xbar, vbar;                     // adjoint of the state (state order)
+    DevBuf<float4> xs, vs, abar;                   // per-step adjoint temporaries (sorted order)
+    DevBuf<float> rhobar, bpart, mpart;
+    DevBuf<double> gB, gmu;                        // per-scene accumulators
+    DevBuf<float> gbuf;                            // 3n staging for gradients
+};
+
+}  // namespace sph
+
+struct sph_handle {
+    // Configuration.
+    std::vector<sph_params> params;        // per scene
+    std::vector<uint32_t> scene_n;         // particles per scene
+    std::vector<uint32_t> scene_off;       // nscenes + 1
+    std::vector<sph::SceneConst> sc_h;
+    bool batched = false;
+    uint32_t n = 0;
+    uint32_t ncells = 0;
+    int passes = 1;
+    bool ready = false;
+    bool has_particles = false;
+    bool has_step = false;
+    bool keys_valid = false;
+    bool keep_acc = false;
+    bool persistent = false;
+    int cu_count = 256;
+    std::string arch;
+    sph::Tuning tune;
+
+    hipStream_t stream = nullptr;
+    // State (state order) and per-step buffers.
+    sph::DevBuf<float4> pos, vel;
+    sph::DevBuf<uint32_t> id, key0;
+    sph::StepBufs work;
+    sph::DevBuf<uint32_t> hist, status, tickets, err;
+    sph::DevBuf<sph::SceneConst> sc_d;
+    std::map<int, sph::DevBuf<uint2>> tiles;
+    std::map<int, uint32_t> ntiles;
+    sph::DevBuf<float> staging;
+
+    // Graph of one step.
+    bool use_graph = true;
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+
+    // Profiling.
+    bool prof = false;
+    std::vector<hipEvent_t> prof_events;   // 6 per recorded step
+    size_t prof_used = 0;
+    double prof_ms[SPH_NUM_PASSES] = {0, 0, 0, 0, 0};
+    uint32_t prof_steps = 0;
+
+    std::unique_ptr<sph::RolloutState> rollout;
+    std::string last_error;
+
+    ~sph_handle();
+};
+
+namespace sph {
+
+int fail(sph_handle* h, int code, const std::string& msg);
+int hip_fail(sph_handle* h, hipError_t e, const char* what);
+int derive_scene(const sph_params& p, SceneConst& sc, std::string& err);
+int alloc_step_bufs(sph_handle* h, StepBufs& w);
+void invalidate_graph(sph_handle* h);
+uint2* tiles_for(sph_handle* h, int block, uint32_t& ntiles);
+uint32_t grid_for(sph_handle* h, int block, uint32_t ntiles);
+StepArgs step_args(sph_handle* h, StepBufs& w, int block);
+// Enqueue one step reading/writing the state buffers with intermediates in w.
+int enqueue_step(sph_handle* h, StepBufs& w, bool record_prof);
+int check_device_error(sph_handle* h);
+int sync(sph_handle* h);
+
+}  // namespace sph

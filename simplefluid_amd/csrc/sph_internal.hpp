@@ -1,0 +1,109 @@
+// Host-side declarations shared by the engine's translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "sph_device.hpp"
+
+namespace sph {
+
+constexpr int kMaxPasses = 4;   // 8-bit digits, keys up to 32 bits
+
+struct SortPlan {
+    uint32_t n = 0;
+    int passes = 1;
+    int items = 8;                    // keys per thread of a digit-pass tile
+    const uint32_t* keys_in = nullptr;  // unsorted keys; values are 0..n-1
+    uint32_t* kbuf[2] = {nullptr, nullptr};
+    uint32_t* vbuf[2] = {nullptr, nullptr};
+    uint32_t* hist = nullptr;         // kMaxPasses x 256, zero on entry
+    uint32_t* status = nullptr;       // passes x tiles x 256
+    uint32_t* tickets = nullptr;      // kMaxPasses
+    uint32_t* err = nullptr;
+    const uint32_t* prev_sorted = nullptr;  // previous step's sorted keys (cell clear)
+    uint32_t prev_n = 0;
+    uint2* cells = nullptr;
+};
+
+uint32_t sort_tiles(uint32_t n, int items);
+int sort_key_passes(uint32_t ncells);
+hipError_t launch_radix_sort(hipStream_t st, const SortPlan& sp);
+// Buffer holding the sorted keys / permutation after launch_radix_sort.
+inline int sort_final(int passes) { return (passes - 1) & 1; }
+
+// Arguments of the per-particle passes.  Work is split in tiles of blockDim
+// particles that never straddle two scenes (tiles[t] = (scene, first)).
+struct StepArgs {
+    const float4* pos_s;     // sorted positions
+    const float4* vel_s;     // sorted velocities
+    float4* pos_o;           // state out (integrate)
+    float4* vel_o;
+    const uint32_t* id_s;
+    uint32_t* id_o;
+    uint32_t* key0;          // next step's unsorted keys
+    const uint2* cells;
+    float* rho;
+    float2* aux;
+    uint32_t* mask;
+    float4* acc;             // optional: acceleration (sorted order)
+    const SceneConst* scenes;
+    const uint2* tiles;
+    uint32_t ntiles;
+    uint32_t n;
+};
+
+hipError_t launch_hash(hipStream_t st, int block, uint32_t grid, const float4* pos, uint32_t* key0,
+                       const SceneConst* scenes, const uint2* tiles, uint32_t ntiles);
+hipError_t launch_reorder(hipStream_t st, int block, uint32_t n, const uint32_t* perm,
+                          const uint32_t* skey, const float4* pos_in, const float4* vel_in,
+                          const uint32_t* id_in, float4* pos_out, float4* vel_out, uint32_t* id_out,
+                          uint2* cells, uint32_t ncells, uint32_t* hist_zero, uint32_t* err);
+hipError_t launch_density(hipStream_t st, int block, uint32_t grid, const StepArgs& a);
+hipError_t launch_force(hipStream_t st, int block, uint32_t grid, const StepArgs& a);
+hipError_t launch_fill_u32(hipStream_t st, uint32_t* p, uint32_t value, size_t count);
+hipError_t launch_fill_cells(hipStream_t st, uint2* cells, size_t count);
+// Debug: neighbour CSR as the force pass sees it.
+hipError_t launch_neighbors(hipStream_t st, int block, uint32_t grid, const StepArgs& a,
+                            const uint32_t* offsets, uint32_t* counts_or_list, int fill);
+// Original-order gather for get_*: out[3*id[k]+c] = src[k].c
+hipError_t launch_unpermute3(hipStream_t st, uint32_t n, const float4* src, const uint32_t* id,
+                             float* out);
+hipError_t launch_unpermute1(hipStream_t st, uint32_t n, const float* src, const uint32_t* id,
+                             float* out);
+// dst[k] = (src[3k], src[3k+1], src[3k+2], 0), or zero when src is null.
+hipError_t launch_pack3(hipStream_t st, uint32_t n, const float* src, float4* dst);
+
+// Reverse pass of one step over a taped step (sorted order of that step).
+struct AdjArgs {
+    const float4* pos_s;
+    const float4* vel_s;
+    const float4* acc;
+    const float* rho;
+    const float2* aux;
+    const uint32_t* mask;
+    const uint2* cells;
+    const uint32_t* perm;
+    float4* xbar;            // in: adjoint of the step output; out: of its input
+    float4* vbar;
+    float4* xs;
+    float4* vs;
+    float4* abar;
+    float* rhobar;
+    float* bpart;            // per-particle dL/dB term
+    float* mpart;            // per-particle dL/dmu term
+    const SceneConst* scenes;
+    const uint2* tiles;
+    uint32_t ntiles;
+    uint32_t n;
+};
+
+hipError_t launch_adj_integrate(hipStream_t st, int block, uint32_t grid, const AdjArgs& a);
+hipError_t launch_adj_force(hipStream_t st, int block, uint32_t grid, const AdjArgs& a);
+hipError_t launch_adj_density(hipStream_t st, int block, uint32_t grid, const AdjArgs& a);
+hipError_t launch_scene_reduce(hipStream_t st, uint32_t nscenes, const SceneConst* scenes,
+                               const float* part, double* acc);
+// out[k] = (g[3 id[k]], g[3 id[k]+1], g[3 id[k]+2], 0)
+hipError_t launch_gather3(hipStream_t st, uint32_t n, const float* g, const uint32_t* id, float4* out);
+
+}  // namespace sph

@@ -1,0 +1,357 @@
+// Per-particle passes of one SPH step on gfx950:
+//   hash     cell key of every state position (first step only; afterwards
+//            the integrate pass writes the next keys)
+//   reorder  gather state into sorted order, write cell start/end
+//   density  rho, EOS, neighbour bit masks (candidate tests happen here once)
+//   force    pressure + viscosity from the masks, integrate, clamp, next key
+// Neighbour order is canonical: rows (dz, dy) = (-1..1, -1..1), dz outer; a
+// row is the contiguous sorted range of cells cx-1..cx+1; ascending index.
+// Density sums and force sums run in that order one particle per lane.
+#include "sph_internal.hpp"
+
+namespace sph {
+
+namespace {
+
+// Tiles handled by this block: a contiguous run per block, and contiguous
+// blocks per XCD (persistent launches use gridDim < ntiles).
+__device__ __forceinline__ void tile_span(uint32_t ntiles, uint32_t& t0, uint32_t& t1) {
+    const uint32_t lb = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t per = (ntiles + gridDim.x - 1) / gridDim.x;
+    t0 = lb * per;
+    t1 = t0 + per < ntiles ? t0 + per : ntiles;
+}
+
+__device__ __forceinline__ void density_one(const StepArgs& a, const SceneConst& sc, uint32_t k) {
+    const float4 pi = a.pos_s[k];
+    const int cx = cell_coord(pi.x, sc.bmin[0], sc.inv_cs, sc.G[0]);
+    const int cy = cell_coord(pi.y, sc.bmin[1], sc.inv_cs, sc.G[1]);
+    const int cz = cell_coord(pi.z, sc.bmin[2], sc.inv_cs, sc.G[2]);
+    const float h2 = sc.h2;
+    float sum = 0.0f;
+#pragma unroll
+    for (int row = 0; row < kRows; ++row) {
+        const int dz = row / 3 - 1, dy = row % 3 - 1;
+        uint32_t lo, hi;
+        row_range(a.cells, sc, cx, cy + dy, cz + dz, lo, hi);
+        uint32_t m = 0u;
+        for (uint32_t j = lo; j < hi; ++j) {
+            const float4 pj = a.pos_s[j];
+            const float d2 = dist2(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z);
+            const bool hit = d2 < h2;
+            const float t = hit ? h2 - d2 : 0.0f;
+            sum = fmaf(t * t, t, sum);
+            const uint32_t b = j - lo;
+            if (hit && b < (uint32_t)kMaskBits) m |= 1u << b;
+        }
+        a.mask[(size_t)row * a.n + k] = m;
+    }
+    const float rho = sc.cD * sum;
+    const float p = eos(sc, rho);
+    const float ir = 1.0f / rho;
+    a.rho[k] = rho;
+    a.aux[k] = make_float2((p * ir) * ir, ir);
+}
+
+struct ForceAcc {
+    float px, py, pz, vx, vy, vz;
+};
+
+__device__ __forceinline__ void force_pair(const StepArgs& a, const SceneConst& sc, const float4& pi,
+                                           const float4& vi, float pti, uint32_t j, ForceAcc& f) {
+    const float4 pj = a.pos_s[j];
+    const float4 vj = a.vel_s[j];
+    const float2 xj = a.aux[j];
+    const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+    const float d2 = dist2(dx, dy, dz);
+    const float inv_r = d2 > 0.0f ? rsqrtf(d2) : 0.0f;
+    const float r = d2 * inv_r;
+    const float hr = sc.h - r;
+    const float w = hr * hr;
+    const float s = ((pti + xj.x) * w) * inv_r;
+    f.px = fmaf(s, dx, f.px);
+    f.py = fmaf(s, dy, f.py);
+    f.pz = fmaf(s, dz, f.pz);
+    const float vv = hr * xj.y;
+    f.vx = fmaf(vv, vj.x - vi.x, f.vx);
+    f.vy = fmaf(vv, vj.y - vi.y, f.vy);
+    f.vz = fmaf(vv, vj.z - vi.z, f.vz);
+}
+
+__device__ __forceinline__ void force_one(const StepArgs& a, const SceneConst& sc, uint32_t k) {
+    const float4 pi = a.pos_s[k];
+    const float4 vi = a.vel_s[k];
+    const float2 xi = a.aux[k];
+    const int cx = cell_coord(pi.x, sc.bmin[0], sc.inv_cs, sc.G[0]);
+    const int cy = cell_coord(pi.y, sc.bmin[1], sc.inv_cs, sc.G[1]);
+    const int cz = cell_coord(pi.z, sc.bmin[2], sc.inv_cs, sc.G[2]);
+    ForceAcc f = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int row = 0; row < kRows; ++row) {
+        const int dz = row / 3 - 1, dy = row % 3 - 1;
+        uint32_t lo, hi;
+        row_range(a.cells, sc, cx, cy + dy, cz + dz, lo, hi);
+        uint32_t m = a.mask[(size_t)row * a.n + k];
+        if (row == 4) {
+            const uint32_t b = k - lo;
+            if (b < (uint32_t)kMaskBits) m &= ~(1u << b);
+        }
+        while (m) {
+            const uint32_t b = (uint32_t)__builtin_ctz(m);
+            m &= m - 1u;
+            force_pair(a, sc, pi, vi, xi.x, lo + b, f);
+        }
+        if (hi > lo + (uint32_t)kMaskBits) {
+            for (uint32_t j = lo + (uint32_t)kMaskBits; j < hi; ++j) {
+                if (j == k) continue;
+                const float4 pj = a.pos_s[j];
+                if (dist2(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z) < sc.h2) force_pair(a, sc, pi, vi, xi.x, j, f);
+            }
+        }
+    }
+    const float cvi = sc.cV * xi.y;
+    const float a0 = (sc.cP * f.px + cvi * f.vx) + sc.g[0];
+    const float a1 = (sc.cP * f.py + cvi * f.vy) + sc.g[1];
+    const float a2 = (sc.cP * f.pz + cvi * f.vz) + sc.g[2];
+    if (a.acc) a.acc[k] = make_float4(a0, a1, a2, 0.0f);
+    float v[3] = {vi.x + sc.dt * a0, vi.y + sc.dt * a1, vi.z + sc.dt * a2};
+    float p[3] = {pi.x + sc.dt * v[0], pi.y + sc.dt * v[1], pi.z + sc.dt * v[2]};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        if (p[d] < sc.lo[d]) {
+            p[d] = sc.lo[d];
+            if (v[d] < 0.0f) v[d] = -sc.e * v[d];
+        } else if (p[d] > sc.hi[d]) {
+            p[d] = sc.hi[d];
+            if (v[d] > 0.0f) v[d] = -sc.e * v[d];
+        }
+    }
+    a.pos_o[k] = make_float4(p[0], p[1], p[2], 0.0f);
+    a.vel_o[k] = make_float4(v[0], v[1], v[2], 0.0f);
+    a.id_o[k] = a.id_s[k];
+    a.key0[k] = cell_key(sc, p[0], p[1], p[2]);
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void density_kernel(StepArgs a) {
+    uint32_t t0, t1;
+    tile_span(a.ntiles, t0, t1);
+    for (uint32_t t = t0; t < t1; ++t) {
+        const uint2 ti = a.tiles[t];
+        const SceneConst& sc = a.scenes[ti.x];
+        const uint32_t k = ti.y + threadIdx.x;
+        if (k < sc.p_end) density_one(a, sc, k);
+    }
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void force_kernel(StepArgs a) {
+    uint32_t t0, t1;
+    tile_span(a.ntiles, t0, t1);
+    for (uint32_t t = t0; t < t1; ++t) {
+        const uint2 ti = a.tiles[t];
+        const SceneConst& sc = a.scenes[ti.x];
+        const uint32_t k = ti.y + threadIdx.x;
+        if (k < sc.p_end) force_one(a, sc, k);
+    }
+}
+
+__global__ void hash_kernel(const float4* __restrict__ pos, uint32_t* __restrict__ key0,
+                            const SceneConst* __restrict__ scenes, const uint2* __restrict__ tiles,
+                            uint32_t ntiles) {
+    uint32_t t0, t1;
+    tile_span(ntiles, t0, t1);
+    for (uint32_t t = t0; t < t1; ++t) {
+        const uint2 ti = tiles[t];
+        const SceneConst& sc = scenes[ti.x];
+        const uint32_t k = ti.y + threadIdx.x;
+        if (k < sc.p_end) {
+            const float4 p = pos[k];
+            key0[k] = cell_key(sc, p.x, p.y, p.z);
+        }
+    }
+}
+
+__global__ void reorder_kernel(uint32_t n, const uint32_t* __restrict__ perm,
+                               const uint32_t* __restrict__ skey, const float4* __restrict__ pos_in,
+                               const float4* __restrict__ vel_in, const uint32_t* __restrict__ id_in,
+                               float4* __restrict__ pos_out, float4* __restrict__ vel_out,
+                               uint32_t* __restrict__ id_out, uint2* __restrict__ cells,
+                               uint32_t ncells, uint32_t* __restrict__ hist_zero, uint32_t* __restrict__ err) {
+    if (hist_zero && blockIdx.x == 0)
+        for (uint32_t i = threadIdx.x; i < (uint32_t)(kMaxPasses * 256); i += blockDim.x) hist_zero[i] = 0u;
+    const uint32_t k = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t s = perm[k];
+    const uint32_t key = skey[k];
+    if (s >= n || key >= ncells) {  // a broken sort raises the flag instead of faulting
+        atomicOr(err, 4u);
+        return;
+    }
+    pos_out[k] = pos_in[s];
+    vel_out[k] = vel_in[s];
+    id_out[k] = id_in[s];
+    if (k == 0 || skey[k - 1] != key) cells[key].x = k;
+    if (k + 1 == n || skey[k + 1] != key) cells[key].y = k + 1;
+}
+
+// Neighbour sets exactly as force_one iterates them (self included).
+template <int FILL>
+__global__ void neighbors_kernel(StepArgs a, const uint32_t* __restrict__ offsets,
+                                 uint32_t* __restrict__ out) {
+    uint32_t t0, t1;
+    tile_span(a.ntiles, t0, t1);
+    for (uint32_t t = t0; t < t1; ++t) {
+        const uint2 ti = a.tiles[t];
+        const SceneConst& sc = a.scenes[ti.x];
+        const uint32_t k = ti.y + threadIdx.x;
+        if (k >= sc.p_end) continue;
+        const float4 pi = a.pos_s[k];
+        const int cx = cell_coord(pi.x, sc.bmin[0], sc.inv_cs, sc.G[0]);
+        const int cy = cell_coord(pi.y, sc.bmin[1], sc.inv_cs, sc.G[1]);
+        const int cz = cell_coord(pi.z, sc.bmin[2], sc.inv_cs, sc.G[2]);
+        uint32_t c = 0;
+        const uint32_t o = FILL ? offsets[k] : 0u;
+        for (int row = 0; row < kRows; ++row) {
+            const int dz = row / 3 - 1, dy = row % 3 - 1;
+            uint32_t lo, hi;
+            row_range(a.cells, sc, cx, cy + dy, cz + dz, lo, hi);
+            uint32_t m = a.mask[(size_t)row * a.n + k];
+            while (m) {
+                const uint32_t b = (uint32_t)__builtin_ctz(m);
+                m &= m - 1u;
+                if (FILL) out[o + c] = lo + b;
+                ++c;
+            }
+            if (hi > lo + (uint32_t)kMaskBits)
+                for (uint32_t j = lo + (uint32_t)kMaskBits; j < hi; ++j) {
+                    const float4 pj = a.pos_s[j];
+                    if (dist2(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z) < sc.h2) {
+                        if (FILL) out[o + c] = j;
+                        ++c;
+                    }
+                }
+        }
+        if (!FILL) out[k] = c;
+    }
+}
+
+__global__ void fill_u32_kernel(uint32_t* __restrict__ p, uint32_t v, size_t count) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ void fill_cells_kernel(uint2* __restrict__ c, size_t count) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x)
+        c[i] = make_uint2(0xffffffffu, 0u);
+}
+
+__global__ void unpermute3_kernel(uint32_t n, const float4* __restrict__ src,
+                                  const uint32_t* __restrict__ id, float* __restrict__ out) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const float4 v = src[k];
+    const uint32_t i = id[k];
+    out[3 * (size_t)i] = v.x;
+    out[3 * (size_t)i + 1] = v.y;
+    out[3 * (size_t)i + 2] = v.z;
+}
+
+__global__ void unpermute1_kernel(uint32_t n, const float* __restrict__ src,
+                                  const uint32_t* __restrict__ id, float* __restrict__ out) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    out[id[k]] = src[k];
+}
+
+__global__ void pack3_kernel(uint32_t n, const float* __restrict__ src, float4* __restrict__ dst) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    dst[k] = src ? make_float4(src[3 * (size_t)k], src[3 * (size_t)k + 1], src[3 * (size_t)k + 2], 0.0f)
+                 : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+inline uint32_t blocks_for(size_t n, int b) { return (uint32_t)((n + b - 1) / b); }
+
+}  // namespace
+
+#define SPH_DISPATCH_BLOCK(KERNEL, block, grid, st, ...)                                       \
+    switch (block) {                                                                          \
+        case 64: hipLaunchKernelGGL(KERNEL<64>, dim3(grid), dim3(64), 0, st, __VA_ARGS__); break;   \
+        case 128: hipLaunchKernelGGL(KERNEL<128>, dim3(grid), dim3(128), 0, st, __VA_ARGS__); break; \
+        case 512: hipLaunchKernelGGL(KERNEL<512>, dim3(grid), dim3(512), 0, st, __VA_ARGS__); break; \
+        default: hipLaunchKernelGGL(KERNEL<256>, dim3(grid), dim3(256), 0, st, __VA_ARGS__); break;  \
+    }
+
+hipError_t launch_density(hipStream_t st, int block, uint32_t grid, const StepArgs& a) {
+    if (grid == 0) return hipSuccess;
+    SPH_DISPATCH_BLOCK(density_kernel, block, grid, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_force(hipStream_t st, int block, uint32_t grid, const StepArgs& a) {
+    if (grid == 0) return hipSuccess;
+    SPH_DISPATCH_BLOCK(force_kernel, block, grid, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_hash(hipStream_t st, int block, uint32_t grid, const float4* pos, uint32_t* key0,
+                       const SceneConst* scenes, const uint2* tiles, uint32_t ntiles) {
+    if (grid == 0) return hipSuccess;
+    hipLaunchKernelGGL(hash_kernel, dim3(grid), dim3(block), 0, st, pos, key0, scenes, tiles, ntiles);
+    return hipGetLastError();
+}
+
+hipError_t launch_reorder(hipStream_t st, int block, uint32_t n, const uint32_t* perm,
+                          const uint32_t* skey, const float4* pos_in, const float4* vel_in,
+                          const uint32_t* id_in, float4* pos_out, float4* vel_out, uint32_t* id_out,
+                          uint2* cells, uint32_t ncells, uint32_t* hist_zero, uint32_t* err) {
+    const uint32_t grid = blocks_for(n, block) > 0 ? blocks_for(n, block) : 1;
+    hipLaunchKernelGGL(reorder_kernel, dim3(grid), dim3(block), 0, st, n, perm, skey, pos_in, vel_in,
+                       id_in, pos_out, vel_out, id_out, cells, ncells, hist_zero, err);
+    return hipGetLastError();
+}
+
+hipError_t launch_neighbors(hipStream_t st, int block, uint32_t grid, const StepArgs& a,
+                            const uint32_t* offsets, uint32_t* out, int fill) {
+    if (grid == 0) return hipSuccess;
+    if (fill)
+        hipLaunchKernelGGL(neighbors_kernel<1>, dim3(grid), dim3(block), 0, st, a, offsets, out);
+    else
+        hipLaunchKernelGGL(neighbors_kernel<0>, dim3(grid), dim3(block), 0, st, a, offsets, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(hipStream_t st, uint32_t* p, uint32_t value, size_t count) {
+    if (count == 0) return hipSuccess;
+    const uint32_t g = blocks_for(count, 256) < 4096 ? blocks_for(count, 256) : 4096;
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(g), dim3(256), 0, st, p, value, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_cells(hipStream_t st, uint2* cells, size_t count) {
+    if (count == 0) return hipSuccess;
+    const uint32_t g = blocks_for(count, 256) < 4096 ? blocks_for(count, 256) : 4096;
+    hipLaunchKernelGGL(fill_cells_kernel, dim3(g), dim3(256), 0, st, cells, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_unpermute3(hipStream_t st, uint32_t n, const float4* src, const uint32_t* id, float* out) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(unpermute3_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, st, n, src, id, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_unpermute1(hipStream_t st, uint32_t n, const float* src, const uint32_t* id, float* out) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(unpermute1_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, st, n, src, id, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack3(hipStream_t st, uint32_t n, const float* src, float4* dst) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(pack3_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, st, n, src, dst);
+    return hipGetLastError();
+}
+
+}  // namespace sph

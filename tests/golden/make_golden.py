@@ -1,0 +1,41 @@
+"""Writes tests/golden/tiny_step.npz: one FP64 oracle step of a small jittered
+lattice (inputs, permutation, neighbour CSR, density, acceleration).
+
+The reference (ttnghia/SimpleFluid) holds no vectors for its solver, which is
+not vendored; these fixtures freeze the restatement once it passed the
+closed-form, brute-force and differentiation checks in tests/test_oracle.py.
+Run: python tests/golden/make_golden.py
+"""
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+import oracle  # noqa: E402
+
+
+def main():
+    rng = np.random.default_rng(2024)
+    r = 0.005
+    i, j, k = np.meshgrid(np.arange(7), np.arange(9), np.arange(6), indexing="ij")
+    ijk = np.stack([i.ravel(), j.ravel(), k.ravel()], 1)
+    pos = (0.2 + r + 2 * r * ijk + rng.uniform(-0.3 * r, 0.3 * r, ijk.shape)).astype(np.float32).astype(np.float64)
+    vel = rng.normal(0, 0.3, pos.shape).astype(np.float32).astype(np.float64)
+    B, mu = 400.0, 0.05
+    o = oracle.Oracle(oracle.make_params(pressure_stiffness=B, viscosity=mu), fp64=True)
+    o.record_neighbors(True)
+    o.set_state(pos, vel)
+    o.step()
+    d = o.step_debug()
+    off, lst = o.neighbors()
+    out = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tiny_step.npz")
+    np.savez(out, pos=pos, vel=vel, B=B, mu=mu, perm=d["perm"], nbr_off=off, nbr_list=lst,
+             rho=d["rho"], acc=d["acc"])
+    print("wrote", out, len(pos), "particles,", len(lst), "neighbour entries")
+
+
+if __name__ == "__main__":
+    main()
