@@ -55,7 +55,9 @@ def plain(args):
     e = engine.Engine(cfg["params"])
     e.set_particles(cfg["pos"])
     e.make_ready()
-    if not args.no_autotune:
+    if args.tuning:
+        e.set_tuning(*[int(x) for x in args.tuning.split(",")])
+    elif not args.no_autotune:
         e.autotune()
     log("tuning", e.get_tuning())
     e.advance_steps(args.warmup)
@@ -191,6 +193,18 @@ def adjoint(args):
             "backward_s": round(tb, 5), "ratio": round(tb / tf, 3)}
 
 
+def calibrate_copy():
+    """A plain device copy of 256 MiB (reads 256 MiB, writes 256 MiB): the
+    reference point that scales FETCH_SIZE / WRITE_SIZE to HBM bytes."""
+    import torch
+    x = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(5):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    log("copy calibration: 5 x 268435456 B read + written")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -201,7 +215,14 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--tuning", type=str, default="",
+                    help="density,force,reorder blocks and sort items, e.g. 512,256,64,16 (skips autotune)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run a 256 MiB device copy (known bytes) for counter calibration, then exit")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported by this build "
               "(scenes shard with no communication; see the design notes)", file=sys.stderr)
