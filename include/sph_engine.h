@@ -142,8 +142,8 @@ int  sph_rollout_backward(sph_handle* h, const float* dL_dx_final, float* dL_dst
 
 /* ---- measurement ---- */
 #define SPH_PASS_HASH      0   /* standalone cell hash (first step only)        */
-#define SPH_PASS_SORT      1   /* radix histogram + digit passes                */
-#define SPH_PASS_REORDER   2   /* gather into sorted order + cell start/end     */
+#define SPH_PASS_SORT      1   /* cell-start scan + sort of keys                */
+#define SPH_PASS_REORDER   2   /* gather of the state into sorted order         */
 #define SPH_PASS_DENSITY   3   /* density, EOS, neighbour masks                 */
 #define SPH_PASS_FORCE     4   /* pressure + viscosity + integrate + clamp      */
 #define SPH_NUM_PASSES     5
@@ -160,6 +160,15 @@ int  sph_autotune(sph_handle* h);
  * per thread (4 ints).                                                      */
 int  sph_get_tuning(const sph_handle* h, int* cfg4);
 int  sph_set_tuning(sph_handle* h, const int* cfg4);
+/* Sort used to order particles by cell (no reference counterpart; both give
+ * the same stable permutation and cell-start table):
+ *   SPH_SORT_CELL_COUNT  one-digit (counting) radix sort over the whole cell
+ *                        key; its histogram scan is the cell-start table (default)
+ *   SPH_SORT_RADIX       multi-pass 8-bit LSD radix sort with decoupled look-back */
+#define SPH_SORT_CELL_COUNT 0
+#define SPH_SORT_RADIX      1
+int  sph_set_sort_mode(sph_handle* h, int mode);
+int  sph_get_sort_mode(const sph_handle* h);
 
 /* ---- test-only views (parity against the CPU oracle) ---- */
 /* Derived constants of scene s: h, h2, inv_cs, mass, cD, cP, cV, inv_rho0,

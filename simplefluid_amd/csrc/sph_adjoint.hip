@@ -42,7 +42,7 @@ __device__ __forceinline__ void for_each_neighbor(const AdjArgs& a, const SceneC
     for (int row = 0; row < kRows; ++row) {
         const int dz = row / 3 - 1, dy = row % 3 - 1;
         uint32_t lo, hi;
-        row_range(a.cells, sc, cx, cy + dy, cz + dz, lo, hi);
+        row_range(a.cstart, sc, cx, cy + dy, cz + dz, lo, hi);
         uint32_t m = a.mask[(size_t)row * a.n + k];
         if (row == 4) {
             const uint32_t b = k - lo;
@@ -115,7 +115,7 @@ __global__ void adj_force_kernel(AdjArgs a) {
             const float2 xj = a.aux[j];
             const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
             const float d2 = dist2(rx, ry, rz);
-            const float inv_r = d2 > 0.0f ? rsqrtf(d2) : 0.0f;
+            const float inv_r = d2 > 0.0f ? __builtin_amdgcn_rsqf(d2) : 0.0f;
             const float dd = d2 * inv_r;
             const float hr = sc.h - dd;
             const float tt = sc.h2 - d2;

@@ -5,7 +5,8 @@
 //   id[2]           uint32 x N   original particle id, state / sorted order
 //   key0            uint32 x N   cell key of the state positions (written by integrate)
 //   sort ping-pong  2 x (uint32 key + uint32 value) x N
-//   cells           uint2  x C   (start, end) of each cell in sorted order; empty = (~0, 0)
+//   count           uint32 x C   particles per cell (atomic, built by the integrate pass)
+//   cstart          uint32 x C+1 lower-bound cell-start table: cstart[c] = #particles with key < c
 //   rho             float  x N   density (sorted order)
 //   aux             float2 x N   (p/rho^2, 1/rho) read by the force pass
 //   mask            uint32 x 9N  neighbour bits, row-major [row][particle]: bit b of row r
@@ -68,22 +69,20 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t bid, uint32_t nblocks) {
     return x * q + (x < r ? x : r) + i;
 }
 
-// Sorted-index range covering cells cx-1..cx+1 of grid row (y, z): cells of a
-// row are contiguous in sorted order, so the union of their ranges is one
-// interval.  Returns lo >= hi for an empty row.
-__device__ __forceinline__ void row_range(const uint2* __restrict__ cells, const SceneConst& sc,
+// Sorted-index range covering cells cx-1..cx+1 of grid row (y, z).  Cells of
+// a row are contiguous in sorted order and cstart holds a lower bound for
+// every cell (empty ones included), so the row is [cstart[first],
+// cstart[last + 1]).  Rows outside the grid are empty (lo = hi = 0).
+__device__ __forceinline__ void row_range(const uint32_t* __restrict__ cstart, const SceneConst& sc,
                                           int cx, int y, int z, uint32_t& lo, uint32_t& hi) {
-    lo = 0xffffffffu;
+    lo = 0u;
     hi = 0u;
     if (y < 0 || y >= sc.G[1] || z < 0 || z >= sc.G[2]) return;
     const int x0 = cx > 0 ? cx - 1 : 0;
     const int x1 = cx + 1 < sc.G[0] ? cx + 1 : sc.G[0] - 1;
     const uint32_t base = sc.cell_offset + ((uint32_t)z * (uint32_t)sc.G[1] + (uint32_t)y) * (uint32_t)sc.G[0];
-    for (int xx = x0; xx <= x1; ++xx) {
-        const uint2 c = cells[base + (uint32_t)xx];
-        lo = min(lo, c.x);
-        hi = max(hi, c.y);
-    }
+    lo = cstart[base + (uint32_t)x0];
+    hi = cstart[base + (uint32_t)x1 + 1u];
 }
 
 __device__ __forceinline__ uint32_t cell_key(const SceneConst& sc, float x, float y, float z) {

@@ -122,7 +122,9 @@ StepArgs step_args(sph_handle* h, StepBufs& w, int block) {
     a.id_s = w.id_s.p;
     a.id_o = h->id.p;
     a.key0 = h->key0.p;
-    a.cells = w.cells.p;
+    a.rank0 = h->rank0.p;
+    a.count = h->count.p;
+    a.cstart = w.cstart.p;
     a.rho = w.rho.p;
     a.aux = w.aux.p;
     a.mask = w.mask.p;
@@ -140,20 +142,19 @@ int alloc_step_bufs(sph_handle* h, StepBufs& w) {
     chk(w.pos_s.ensure(n));
     chk(w.vel_s.ensure(n));
     chk(w.id_s.ensure(n));
-    for (int i = 0; i < 2; ++i) { chk(w.kbuf[i].ensure(n)); chk(w.vbuf[i].ensure(n)); }
-    const bool fresh_cells = w.cells.count < h->ncells || !w.cells.p;
-    chk(w.cells.ensure(h->ncells));
+    chk(w.skey.ensure(n));
+    chk(w.perm.ensure(n));
+    chk(w.cstart.ensure((size_t)h->ncells + 1));
+    if (h->tune.sort_mode == kSortCellCount) {
+        chk(w.tmp.ensure(n));
+    } else {
+        for (int i = 0; i < 2; ++i) { chk(w.kbuf[i].ensure(n)); chk(w.vbuf[i].ensure(n)); }
+    }
     chk(w.rho.ensure(n));
     chk(w.aux.ensure(n));
     chk(w.mask.ensure((size_t)kRows * n));
     if (h->keep_acc) chk(w.acc.ensure(n));
-    if (e != hipSuccess) return hip_fail(h, e, "device allocation");
-    if (fresh_cells || !w.cells_clean) {
-        chk(launch_fill_cells(h->stream, w.cells.p, h->ncells));
-        w.cells_clean = true;
-        w.prev_valid = false;
-    }
-    return hip_fail(h, e, "cell table clear");
+    return hip_fail(h, e, "device allocation");
 }
 
 static int ensure_buffers(sph_handle* h) {
@@ -164,15 +165,22 @@ static int ensure_buffers(sph_handle* h) {
     chk(h->vel.ensure(n));
     chk(h->id.ensure(n));
     chk(h->key0.ensure(n));
+    chk(h->rank0.ensure(n));
+    const bool fresh_count = h->count.count < h->ncells || !h->count.p;
+    chk(h->count.ensure(h->ncells));
     chk(h->hist.ensure(kMaxPasses * 256));
     chk(h->status.ensure((size_t)kMaxPasses * sort_tiles((uint32_t)n, 4) * 256 + 256));
     chk(h->tickets.ensure(kMaxPasses));
+    const bool fresh_scan = h->scan_status.count < scan_tiles(h->ncells) + 1 || !h->scan_status.p;
+    chk(h->scan_status.ensure(scan_tiles(h->ncells) + 1));
     chk(h->err.ensure(1));
     chk(h->staging.ensure(3 * n + 1));
     chk(h->sc_d.ensure(h->sc_h.size()));
     if (e != hipSuccess) return hip_fail(h, e, "device allocation");
     chk(launch_fill_u32(h->stream, h->hist.p, 0u, kMaxPasses * 256));
     chk(launch_fill_u32(h->stream, h->err.p, 0u, 1));
+    if (fresh_count) chk(launch_fill_u32(h->stream, h->count.p, 0u, h->ncells));
+    if (fresh_scan) chk(launch_fill_u32(h->stream, h->scan_status.p, 0u, scan_tiles(h->ncells) + 1));
     chk(hipMemcpyAsync(h->sc_d.p, h->sc_h.data(), h->sc_h.size() * sizeof(SceneConst), hipMemcpyHostToDevice, h->stream));
     if (e != hipSuccess) return hip_fail(h, e, "buffer init");
     return alloc_step_bufs(h, h->work);
@@ -192,35 +200,47 @@ int enqueue_step(sph_handle* h, StepBufs& w, bool record_prof) {
     hipStream_t st = h->stream;
     if (record_prof) prof_record(h, 0);
     if (!h->keys_valid) {
+        // New state: counts start from zero, then every particle is hashed.
+        hipError_t e = launch_fill_u32(st, h->count.p, 0u, h->ncells);
         uint32_t nt = 0;
         const uint2* tl = tiles_for(h, 256, nt);
-        hipError_t e = launch_hash(st, 256, nt, h->pos.p, h->key0.p, h->sc_d.p, tl, nt);
+        if (e == hipSuccess)
+            e = launch_hash(st, 256, nt, h->pos.p, h->key0.p, h->rank0.p, h->count.p, h->sc_d.p, tl, nt);
         if (e != hipSuccess) return hip_fail(h, e, "hash launch");
         h->keys_valid = true;
     }
     if (record_prof) prof_record(h, 1);
     SortPlan sp;
     sp.n = h->n;
+    sp.ncells = h->ncells;
+    sp.mode = h->tune.sort_mode;
     sp.passes = h->passes;
     sp.items = h->tune.sort_items;
-    sp.keys_in = h->key0.p;
+    sp.reorder_block = h->tune.reorder_block;
+    sp.key0 = h->key0.p;
+    sp.rank0 = h->rank0.p;
+    sp.count = h->count.p;
+    sp.cstart = w.cstart.p;
+    sp.tmp = w.tmp.p;
     for (int i = 0; i < 2; ++i) { sp.kbuf[i] = w.kbuf[i].p; sp.vbuf[i] = w.vbuf[i].p; }
+    sp.skey = w.skey.p;
+    sp.perm = w.perm.p;
     sp.hist = h->hist.p;
     sp.status = h->status.p;
     sp.tickets = h->tickets.p;
+    sp.scan_status = h->scan_status.p;
     sp.err = h->err.p;
-    const int fin = sort_final(h->passes);
-    sp.prev_sorted = w.kbuf[fin].p;
-    sp.prev_n = w.prev_valid ? h->n : 0u;
-    sp.cells = w.cells.p;
-    hipError_t e = launch_radix_sort(st, sp);
+    sp.pos_in = h->pos.p;
+    sp.vel_in = h->vel.p;
+    sp.id_in = h->id.p;
+    sp.pos_out = w.pos_s.p;
+    sp.vel_out = w.vel_s.p;
+    sp.id_out = w.id_s.p;
+    hipError_t e = launch_sort_keys(st, sp);
     if (e != hipSuccess) return hip_fail(h, e, "sort launch");
     if (record_prof) prof_record(h, 2);
-    e = launch_reorder(st, h->tune.reorder_block, h->n, w.vbuf[fin].p, w.kbuf[fin].p, h->pos.p, h->vel.p,
-                       h->id.p, w.pos_s.p, w.vel_s.p, w.id_s.p, w.cells.p, h->ncells, h->hist.p, h->err.p);
+    e = launch_sort_gather(st, sp);
     if (e != hipSuccess) return hip_fail(h, e, "reorder launch");
-    w.prev_valid = true;
-    w.cells_clean = false;
     if (record_prof) prof_record(h, 3);
     {
         StepArgs a = step_args(h, w, h->tune.density_block);
@@ -249,7 +269,7 @@ int check_device_error(sph_handle* h) {
         std::string what;
         if (flag & 1u) what += " sort look-back wait exceeded its bound;";
         if (flag & 2u) what += " sort scatter index out of range;";
-        if (flag & 4u) what += " reorder index out of range;";
+        if (flag & 4u) what += " sort gather index out of range;";
         return fail(h, SPH_E_DEVICE, "device error flag set:" + what);
     }
     return SPH_OK;
@@ -423,12 +443,6 @@ static int upload_state(sph_handle* h, const float* xyz, const float* vel, const
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     if (e != hipSuccess) return hip_fail(h, e, "state upload");
     h->keys_valid = false;
-    h->work.prev_valid = false;
-    if (!h->work.cells_clean) {
-        e = launch_fill_cells(h->stream, h->work.cells.p, h->ncells);
-        if (e != hipSuccess) return hip_fail(h, e, "cell table clear");
-        h->work.cells_clean = true;
-    }
     invalidate_graph(h);
     return SPH_OK;
 }
@@ -498,7 +512,7 @@ int sph_advance_steps(sph_handle* h, uint32_t k) {
     for (uint32_t i = 0; i < k; ++i) {
         // The graph holds a steady-state step: keys from the last integrate,
         // a cell table to clear sparsely.  Earlier steps run eagerly.
-        const bool steady = h->keys_valid && h->work.prev_valid && !h->keep_acc;
+        const bool steady = h->keys_valid && !h->keep_acc;
         if (h->use_graph && steady && !h->prof) {
             if (!h->gexec) {
                 int rc = capture_graph(h);
@@ -680,6 +694,22 @@ int sph_set_tuning(sph_handle* h, const int* cfg) {
     return SPH_OK;
 }
 
+int sph_set_sort_mode(sph_handle* h, int mode) {
+    if (!h) return SPH_E_INVALID;
+    if (mode != SPH_SORT_CELL_COUNT && mode != SPH_SORT_RADIX) return fail(h, SPH_E_INVALID, "unknown sort mode");
+    if (mode == SPH_SORT_RADIX && h->passes > kMaxPasses) return fail(h, SPH_E_INVALID, "key space too large");
+    h->tune.sort_mode = mode;
+    invalidate_graph(h);
+    if (h->n) {
+        const int rc = alloc_step_bufs(h, h->work);
+        if (rc != SPH_OK) return rc;
+    }
+    return SPH_OK;
+}
+
+int sph_get_sort_mode(const sph_handle* h) { return h ? h->tune.sort_mode : SPH_E_INVALID;
+}
+
 // ---- test-only views ----
 
 int sph_debug_constants(const sph_handle* h, uint32_t scene, float* out14, int* G3) {
@@ -752,11 +782,9 @@ int sph_debug_step_outputs(sph_handle* h, uint32_t* keys_in, uint32_t* perm, uin
     int rc = sync(h);
     if (rc != SPH_OK) return rc;
     const size_t n = h->n;
-    const int fin = sort_final(h->passes);
     hipError_t e = hipSuccess;
-    (void)keys_in;
-    if (perm) e = hipMemcpy(perm, h->work.vbuf[fin].p, n * 4, hipMemcpyDeviceToHost);
-    if (e == hipSuccess && sorted_keys) e = hipMemcpy(sorted_keys, h->work.kbuf[fin].p, n * 4, hipMemcpyDeviceToHost);
+    if (perm) e = hipMemcpy(perm, h->work.perm.p, n * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && sorted_keys) e = hipMemcpy(sorted_keys, h->work.skey.p, n * 4, hipMemcpyDeviceToHost);
     if (e == hipSuccess && rho) e = hipMemcpy(rho, h->work.rho.p, n * 4, hipMemcpyDeviceToHost);
     if (e == hipSuccess && acc3) {
         if (!h->work.acc.p) return fail(h, SPH_E_NOT_READY, "enable sph_debug_keep_acc before stepping");
@@ -767,8 +795,8 @@ int sph_debug_step_outputs(sph_handle* h, uint32_t* keys_in, uint32_t* perm, uin
     if (e == hipSuccess && keys_in) {
         // Unsorted keys of the last step's input: keys_in[perm[k]] = sorted_keys[k].
         std::vector<uint32_t> pm(n), sk(n);
-        e = hipMemcpy(pm.data(), h->work.vbuf[fin].p, n * 4, hipMemcpyDeviceToHost);
-        if (e == hipSuccess) e = hipMemcpy(sk.data(), h->work.kbuf[fin].p, n * 4, hipMemcpyDeviceToHost);
+        e = hipMemcpy(pm.data(), h->work.perm.p, n * 4, hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(sk.data(), h->work.skey.p, n * 4, hipMemcpyDeviceToHost);
         for (size_t k = 0; k < n && e == hipSuccess; ++k) keys_in[pm[k]] = sk[k];
     }
     return hip_fail(h, e, "debug download");
@@ -778,12 +806,18 @@ uint32_t sph_debug_num_cells(const sph_handle* h) { return h ? h->ncells : 0u; }
 
 int sph_debug_cells(sph_handle* h, uint32_t* start, uint32_t* end) {
     if (!h || !start || !end) return SPH_E_INVALID;
+    if (!h->has_step) return fail(h, SPH_E_NOT_READY, "no step has run");
     int rc = sync(h);
     if (rc != SPH_OK) return rc;
-    std::vector<uint2> c(h->ncells);
-    hipError_t e = hipMemcpy(c.data(), h->work.cells.p, c.size() * sizeof(uint2), hipMemcpyDeviceToHost);
+    std::vector<uint32_t> c((size_t)h->ncells + 1);
+    hipError_t e = hipMemcpy(c.data(), h->work.cstart.p, c.size() * sizeof(uint32_t), hipMemcpyDeviceToHost);
     if (e != hipSuccess) return hip_fail(h, e, "cell download");
-    for (size_t i = 0; i < c.size(); ++i) { start[i] = c[i].x; end[i] = c[i].y; }
+    // Report (start, end) per cell; empty cells as (~0, 0).
+    for (size_t i = 0; i < h->ncells; ++i) {
+        const bool empty = c[i] == c[i + 1];
+        start[i] = empty ? 0xffffffffu : c[i];
+        end[i] = empty ? 0u : c[i + 1];
+    }
     return SPH_OK;
 }
 

@@ -38,14 +38,14 @@ struct DevBuf {
 struct StepBufs {
     DevBuf<float4> pos_s, vel_s;   // sorted input state
     DevBuf<uint32_t> id_s;
-    DevBuf<uint32_t> kbuf[2], vbuf[2];
-    DevBuf<uint2> cells;
+    DevBuf<uint32_t> skey, perm;   // sorted keys, sorted index -> state index
+    DevBuf<uint32_t> cstart;       // C + 1 lower-bound cell-start table
+    DevBuf<uint32_t> tmp;          // cell-count sort slots
+    DevBuf<uint32_t> kbuf[2], vbuf[2];  // radix ping-pong (radix mode only)
     DevBuf<float> rho;
     DevBuf<float2> aux;
     DevBuf<uint32_t> mask;
     DevBuf<float4> acc;
-    bool cells_clean = false;      // cells all-empty (fresh)
-    bool prev_valid = false;       // kbuf[final] holds the keys that wrote cells
 };
 
 struct Tuning {
@@ -53,6 +53,7 @@ struct Tuning {
     int force_block = 256;
     int reorder_block = 256;
     int sort_items = 8;
+    int sort_mode = kSortCellCount;
 };
 
 // Differentiable rollout: a checkpoint of the state every `every` steps and
@@ -98,9 +99,9 @@ struct sph_handle {
     hipStream_t stream = nullptr;
     // State (state order) and per-step buffers.
     sph::DevBuf<float4> pos, vel;
-    sph::DevBuf<uint32_t> id, key0;
+    sph::DevBuf<uint32_t> id, key0, rank0, count;
     sph::StepBufs work;
-    sph::DevBuf<uint32_t> hist, status, tickets, err;
+    sph::DevBuf<uint32_t> hist, status, tickets, scan_status, err;
     sph::DevBuf<sph::SceneConst> sc_d;
     std::map<int, sph::DevBuf<uint2>> tiles;
     std::map<int, uint32_t> ntiles;

@@ -10,27 +10,45 @@ namespace sph {
 
 constexpr int kMaxPasses = 4;   // 8-bit digits, keys up to 32 bits
 
+constexpr int kSortCellCount = 0;   // one-digit (counting) radix sort, default
+constexpr int kSortRadix = 1;       // multi-pass 8-bit LSD radix sort
+
 struct SortPlan {
     uint32_t n = 0;
-    int passes = 1;
-    int items = 8;                    // keys per thread of a digit-pass tile
-    const uint32_t* keys_in = nullptr;  // unsorted keys; values are 0..n-1
-    uint32_t* kbuf[2] = {nullptr, nullptr};
+    uint32_t ncells = 0;
+    int mode = kSortCellCount;
+    int passes = 1;                   // radix: 8-bit digits
+    int items = 8;                    // radix: keys per thread of a digit-pass tile
+    int reorder_block = 256;
+    const uint32_t* key0 = nullptr;   // unsorted keys (state order)
+    const uint32_t* rank0 = nullptr;  // slot each particle took in its cell's count
+    uint32_t* count = nullptr;        // C, zeroed by the scan
+    uint32_t* cstart = nullptr;       // C + 1, written by the scan
+    uint32_t* tmp = nullptr;          // cell-count: slot -> particle
+    uint32_t* kbuf[2] = {nullptr, nullptr};  // radix ping-pong
     uint32_t* vbuf[2] = {nullptr, nullptr};
-    uint32_t* hist = nullptr;         // kMaxPasses x 256, zero on entry
-    uint32_t* status = nullptr;       // passes x tiles x 256
-    uint32_t* tickets = nullptr;      // kMaxPasses
+    uint32_t* skey = nullptr;         // sorted keys
+    uint32_t* perm = nullptr;         // sorted index -> state index
+    uint32_t* hist = nullptr;         // radix: kMaxPasses x 256, zero on entry
+    uint32_t* status = nullptr;       // radix: passes x tiles x 256
+    uint32_t* tickets = nullptr;      // radix: kMaxPasses
+    uint32_t* scan_status = nullptr;  // scan tiles + 1 (ticket), zero on entry
     uint32_t* err = nullptr;
-    const uint32_t* prev_sorted = nullptr;  // previous step's sorted keys (cell clear)
-    uint32_t prev_n = 0;
-    uint2* cells = nullptr;
+    const float4* pos_in = nullptr;
+    const float4* vel_in = nullptr;
+    const uint32_t* id_in = nullptr;
+    float4* pos_out = nullptr;
+    float4* vel_out = nullptr;
+    uint32_t* id_out = nullptr;
 };
 
 uint32_t sort_tiles(uint32_t n, int items);
+uint32_t scan_tiles(uint32_t ncells);
 int sort_key_passes(uint32_t ncells);
-hipError_t launch_radix_sort(hipStream_t st, const SortPlan& sp);
-// Buffer holding the sorted keys / permutation after launch_radix_sort.
-inline int sort_final(int passes) { return (passes - 1) & 1; }
+// Cell-start table plus the sort proper (keys and permutation) ...
+hipError_t launch_sort_keys(hipStream_t st, const SortPlan& sp);
+// ... then the gather of the state into sorted order.
+hipError_t launch_sort_gather(hipStream_t st, const SortPlan& sp);
 
 // Arguments of the per-particle passes.  Work is split in tiles of blockDim
 // particles that never straddle two scenes (tiles[t] = (scene, first)).
@@ -42,7 +60,9 @@ struct StepArgs {
     const uint32_t* id_s;
     uint32_t* id_o;
     uint32_t* key0;          // next step's unsorted keys
-    const uint2* cells;
+    uint32_t* rank0;         // next step's slot in its cell count
+    uint32_t* count;         // next step's per-cell counts (atomic)
+    const uint32_t* cstart;  // cell-start table of this step
     float* rho;
     float2* aux;
     uint32_t* mask;
@@ -54,15 +74,11 @@ struct StepArgs {
 };
 
 hipError_t launch_hash(hipStream_t st, int block, uint32_t grid, const float4* pos, uint32_t* key0,
-                       const SceneConst* scenes, const uint2* tiles, uint32_t ntiles);
-hipError_t launch_reorder(hipStream_t st, int block, uint32_t n, const uint32_t* perm,
-                          const uint32_t* skey, const float4* pos_in, const float4* vel_in,
-                          const uint32_t* id_in, float4* pos_out, float4* vel_out, uint32_t* id_out,
-                          uint2* cells, uint32_t ncells, uint32_t* hist_zero, uint32_t* err);
+                       uint32_t* rank0, uint32_t* count, const SceneConst* scenes, const uint2* tiles,
+                       uint32_t ntiles);
 hipError_t launch_density(hipStream_t st, int block, uint32_t grid, const StepArgs& a);
 hipError_t launch_force(hipStream_t st, int block, uint32_t grid, const StepArgs& a);
 hipError_t launch_fill_u32(hipStream_t st, uint32_t* p, uint32_t value, size_t count);
-hipError_t launch_fill_cells(hipStream_t st, uint2* cells, size_t count);
 // Debug: neighbour CSR as the force pass sees it.
 hipError_t launch_neighbors(hipStream_t st, int block, uint32_t grid, const StepArgs& a,
                             const uint32_t* offsets, uint32_t* counts_or_list, int fill);
@@ -82,7 +98,7 @@ struct AdjArgs {
     const float* rho;
     const float2* aux;
     const uint32_t* mask;
-    const uint2* cells;
+    const uint32_t* cstart;
     const uint32_t* perm;
     float4* xbar;            // in: adjoint of the step output; out: of its input
     float4* vbar;

@@ -1,15 +1,27 @@
-// Stable LSD radix sort of (cell key, particle index) for gfx950.
+// Sorting particles by cell key on gfx950, and the cell-start table.
 //
-// One histogram kernel computes the 256-bin digit counts of every pass at
-// once; then one kernel per 8-bit digit ranks a tile of keys in LDS and finds
-// its global offsets by decoupled look-back over earlier tiles (tiles are
-// taken in order from an atomic ticket, so every tile a block waits on is
-// already resident and running).  Ranking inside a wave uses ballot-matching
-// of the eight digit bits across the 64 lanes, so equal digits are ranked in
-// lane order without LDS atomics, which keeps the sort stable.
+// Both sort modes produce the same stable permutation (ties keep the input
+// order) and the same lower-bound table cstart[c] = #particles with key < c.
 //
-// Traffic per pass: read key + value (8 B), write key + value (8 B); the
-// histogram kernel reads each key once for all passes.
+// Cell-count sort (default): a one-digit radix sort whose digit is the whole
+// cell key.  The integrate pass already counted particles per cell with
+// atomics (count[key]++, keeping the returned slot in rank0).  Then
+//   cell_scan        exclusive scan of count -> cstart (decoupled look-back,
+//                    zeroes count for the next step)
+//   cell_scatter     tmp[cstart[key] + rank0] = k   (slot order within a
+//                    cell is whatever order the atomics ran in)
+//   cell_rank        the deterministic rank of k in its cell is the number of
+//                    members with a smaller input index; k is gathered
+//                    straight into its sorted place (pos, vel, id, key, perm)
+// Radix sort: one histogram kernel for all 8-bit digits, then one kernel per
+// digit that ranks a tile of keys in LDS (wave64 ballot-matching of the
+// digit bits, so equal digits rank in lane order without LDS atomics) and
+// finds its global offsets by decoupled look-back; a gather kernel then
+// reorders the state.  It is kept for key spaces too large to count.
+//
+// Look-back waits are bounded: a wait that exceeds its bound raises the
+// device error flag instead of hanging, and every data-dependent store is
+// range-checked.
 #include "sph_internal.hpp"
 
 namespace sph {
@@ -24,6 +36,8 @@ constexpr uint32_t kFlagInc = 2u << 30;
 constexpr uint32_t kValMask = (1u << 30) - 1u;
 constexpr uint32_t kSpinLimit = 1u << 22;
 constexpr int kHistItems = 16;
+constexpr int kScanItems = 16;
+constexpr int kScanTile = kSortBlock * kScanItems;
 
 __device__ __forceinline__ uint32_t ld_agent(const uint32_t* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -32,8 +46,9 @@ __device__ __forceinline__ void st_agent(uint32_t* p, uint32_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Exclusive scan of one value per thread over a 256-thread block.
-__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_w) {
+// Exclusive scan of one value per thread over a 256-thread block; *total
+// receives the block sum.
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s_w, uint32_t* total = nullptr) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     uint32_t x = v;
 #pragma unroll
@@ -43,25 +58,155 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s
     }
     if (lane == 63) s_w[wave] = x;
     __syncthreads();
-    uint32_t pre = 0;
+    uint32_t pre = 0, tot = 0;
 #pragma unroll
-    for (int w = 0; w < kSortWaves; ++w)
-        if (w < wave) pre += s_w[w];
+    for (int w = 0; w < kSortWaves; ++w) {
+        const uint32_t sw = s_w[w];
+        if (w < wave) pre += sw;
+        tot += sw;
+    }
     __syncthreads();
+    if (total) *total = tot;
     return pre + x - v;
 }
 
+// Inclusive prefix of the tiles before `tile` in a look-back status array
+// (one word per tile and digit: 2-bit flag, 30-bit value).
+__device__ __forceinline__ uint32_t look_back(const uint32_t* status, uint32_t stride, uint32_t tile, uint32_t* err) {
+    uint32_t prefix = 0;
+    int t = (int)tile - 1;
+    uint32_t spins = 0;
+    while (t >= 0) {
+        const uint32_t s = ld_agent(status + (size_t)t * stride);
+        const uint32_t f = s & ~kValMask;
+        if (f == 0u) {
+            if (++spins > kSpinLimit) {
+                atomicOr(err, 1u);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+            continue;
+        }
+        prefix += s & kValMask;
+        if (f == kFlagInc) break;
+        --t;
+    }
+    return prefix;
+}
+
+// ---------------------------------------------------------------- cell count
+
+__global__ __launch_bounds__(kSortBlock) void cell_scan_kernel(uint32_t* __restrict__ count,
+                                                               uint32_t* __restrict__ cstart, uint32_t C,
+                                                               uint32_t* status, uint32_t* ticket, uint32_t* err) {
+    __shared__ uint32_t s_tile, s_prefix;
+    __shared__ uint32_t s_w[kSortWaves];
+    const int tid = threadIdx.x;
+    if (tid == 0) s_tile = atomicAdd(ticket, 1u);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const uint32_t first = tile * (uint32_t)kScanTile + (uint32_t)tid * kScanItems;
+    uint32_t v[kScanItems];
+    uint32_t sum = 0;
+    if (first + kScanItems <= C) {
+        const uint4* src = reinterpret_cast<const uint4*>(count + first);
+#pragma unroll
+        for (int q = 0; q < kScanItems / 4; ++q) {
+            const uint4 u = src[q];
+            v[4 * q] = u.x; v[4 * q + 1] = u.y; v[4 * q + 2] = u.z; v[4 * q + 3] = u.w;
+        }
+        uint4* dst = reinterpret_cast<uint4*>(count + first);
+#pragma unroll
+        for (int q = 0; q < kScanItems / 4; ++q) dst[q] = make_uint4(0u, 0u, 0u, 0u);
+    } else {
+#pragma unroll
+        for (int i = 0; i < kScanItems; ++i) {
+            const uint32_t idx = first + i;
+            v[i] = idx < C ? count[idx] : 0u;
+            if (idx < C) count[idx] = 0u;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) sum += v[i];
+    uint32_t total = 0;
+    const uint32_t excl = block_exclusive_scan(sum, s_w, &total);
+    if (tid == 0) {
+        uint32_t prefix = 0;
+        if (tile == 0) {
+            st_agent(status, kFlagInc | total);
+        } else {
+            st_agent(status + tile, kFlagAgg | total);
+            prefix = look_back(status, 1u, tile, err);
+            st_agent(status + tile, kFlagInc | (prefix + total));
+        }
+        s_prefix = prefix;
+    }
+    __syncthreads();
+    uint32_t run = s_prefix + excl;
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+        const uint32_t idx = first + i;
+        if (idx < C) cstart[idx] = run;
+        run += v[i];
+        if (idx + 1 == C) cstart[C] = run;
+    }
+}
+
+__global__ void cell_scatter_kernel(uint32_t n, uint32_t C, const uint32_t* __restrict__ key0,
+                                    const uint32_t* __restrict__ rank0, const uint32_t* __restrict__ cstart,
+                                    uint32_t* __restrict__ tmp, uint32_t* __restrict__ err) {
+    const uint32_t k = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t key = key0[k];
+    const uint32_t slot = key < C ? cstart[key] + rank0[k] : 0xffffffffu;
+    if (slot < n)
+        tmp[slot] = k;
+    else
+        atomicOr(err, 2u);
+}
+
+__global__ void cell_rank_kernel(uint32_t n, uint32_t C, const uint32_t* __restrict__ key0,
+                                 const uint32_t* __restrict__ cstart, const uint32_t* __restrict__ tmp,
+                                 const float4* __restrict__ pos_in, const float4* __restrict__ vel_in,
+                                 const uint32_t* __restrict__ id_in, float4* __restrict__ pos_out,
+                                 float4* __restrict__ vel_out, uint32_t* __restrict__ id_out,
+                                 uint32_t* __restrict__ skey, uint32_t* __restrict__ perm,
+                                 uint32_t* __restrict__ scan_reset, uint32_t scan_reset_words,
+                                 uint32_t* __restrict__ err) {
+    if (blockIdx.x == 0)
+        for (uint32_t i = threadIdx.x; i < scan_reset_words; i += blockDim.x) scan_reset[i] = 0u;
+    const uint32_t k = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t key = key0[k];
+    if (key >= C) {
+        atomicOr(err, 4u);
+        return;
+    }
+    const uint32_t lo = cstart[key], hi = cstart[key + 1];
+    uint32_t r = 0;
+    for (uint32_t t = lo; t < hi; ++t) r += tmp[t] < k ? 1u : 0u;
+    const uint32_t dst = lo + r;
+    if (dst >= hi || hi > n) {
+        atomicOr(err, 4u);
+        return;
+    }
+    perm[dst] = k;
+    skey[dst] = key;
+    pos_out[dst] = pos_in[k];
+    vel_out[dst] = vel_in[k];
+    id_out[dst] = id_in[k];
+}
+
+// --------------------------------------------------------------------- radix
+
 // Digit histograms of all passes; zeroes the look-back status words and the
-// tile tickets of this sort; clears the cell entries the previous step wrote
-// (prev_sorted holds that step's sorted keys, prev_n of them).
+// tile tickets of this sort.
 __global__ __launch_bounds__(kSortBlock) void radix_histogram_kernel(
     const uint32_t* __restrict__ keys, uint32_t n, int passes, uint32_t* __restrict__ hist,
-    uint32_t* __restrict__ status, uint32_t status_words, uint32_t* __restrict__ tickets,
-    const uint32_t* __restrict__ prev_sorted, uint32_t prev_n, uint2* __restrict__ cells) {
+    uint32_t* __restrict__ status, uint32_t status_words, uint32_t* __restrict__ tickets) {
     __shared__ uint32_t s_hist[kMaxPasses * kBins];
     for (int i = threadIdx.x; i < kMaxPasses * kBins; i += kSortBlock) s_hist[i] = 0;
     __syncthreads();
-
     // Each thread counts a contiguous run of keys and only flushes a digit to
     // LDS when it changes: sorted-ish input keeps the high digits constant.
     const uint32_t first = (blockIdx.x * kSortBlock + threadIdx.x) * kHistItems;
@@ -87,15 +232,10 @@ __global__ __launch_bounds__(kSortBlock) void radix_histogram_kernel(
 #pragma unroll
     for (int p = 0; p < kMaxPasses; ++p)
         if (p < passes && cnt[p]) atomicAdd(&s_hist[p * kBins + cur[p]], cnt[p]);
-
     const uint32_t stride = gridDim.x * kSortBlock;
     const uint32_t tid = blockIdx.x * kSortBlock + threadIdx.x;
     for (uint32_t i = tid; i < status_words; i += stride) status[i] = 0u;
     if (tid < (uint32_t)kMaxPasses) tickets[tid] = 0u;
-    for (uint32_t i = tid; i < prev_n; i += stride) {
-        const uint32_t k = prev_sorted[i];
-        if (i == 0 || prev_sorted[i - 1] != k) cells[k] = make_uint2(0xffffffffu, 0u);
-    }
     __syncthreads();
     for (int i = threadIdx.x; i < passes * kBins; i += kSortBlock) {
         const uint32_t v = s_hist[i];
@@ -168,30 +308,12 @@ __global__ __launch_bounds__(kSortBlock) void radix_pass_kernel(
     }
     uint32_t* my = status + (size_t)tile * kBins + tid;
     st_agent(my, (tile == 0 ? kFlagInc : kFlagAgg) | tot);
-
     const uint32_t lstart = block_exclusive_scan(tot, s_w);
     const uint32_t hbase = block_exclusive_scan(hist[tid], s_w);
     s_lstart[tid] = lstart;
-
     uint32_t prefix = 0;
     if (tile > 0) {
-        int t = (int)tile - 1;
-        uint32_t spins = 0;
-        while (true) {
-            const uint32_t s = ld_agent(status + (size_t)t * kBins + tid);
-            const uint32_t f = s & ~kValMask;
-            if (f == 0u) {
-                if (++spins > kSpinLimit) {
-                    atomicOr(err, 1u);
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(1);
-                continue;
-            }
-            prefix += s & kValMask;
-            if (f == kFlagInc || t == 0) break;
-            --t;
-        }
+        prefix = look_back(status + tid, kBins, tile, err);
         st_agent(my, kFlagInc | (prefix + tot));
     }
     s_gbase[tid] = hbase + prefix;
@@ -225,6 +347,30 @@ __global__ __launch_bounds__(kSortBlock) void radix_pass_kernel(
     }
 }
 
+__global__ void radix_reorder_kernel(uint32_t n, const uint32_t* __restrict__ perm,
+                                     const float4* __restrict__ pos_in, const float4* __restrict__ vel_in,
+                                     const uint32_t* __restrict__ id_in, float4* __restrict__ pos_out,
+                                     float4* __restrict__ vel_out, uint32_t* __restrict__ id_out,
+                                     uint32_t* __restrict__ hist_zero, uint32_t* __restrict__ scan_reset,
+                                     uint32_t scan_reset_words, uint32_t* __restrict__ err) {
+    if (blockIdx.x == 0) {
+        for (uint32_t i = threadIdx.x; i < (uint32_t)(kMaxPasses * kBins); i += blockDim.x) hist_zero[i] = 0u;
+        for (uint32_t i = threadIdx.x; i < scan_reset_words; i += blockDim.x) scan_reset[i] = 0u;
+    }
+    const uint32_t k = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t s = perm[k];
+    if (s >= n) {
+        atomicOr(err, 4u);
+        return;
+    }
+    pos_out[k] = pos_in[s];
+    vel_out[k] = vel_in[s];
+    id_out[k] = id_in[s];
+}
+
+inline uint32_t blocks_for(size_t n, int b) { return (uint32_t)((n + b - 1) / b); }
+
 }  // namespace
 
 uint32_t sort_tiles(uint32_t n, int items) {
@@ -232,25 +378,36 @@ uint32_t sort_tiles(uint32_t n, int items) {
     return (n + tile - 1) / tile;
 }
 
+uint32_t scan_tiles(uint32_t C) { return (C + kScanTile - 1) / kScanTile; }
+
 int sort_key_passes(uint32_t ncells) {
     int bits = 1;
     while (bits < 32 && (1ull << bits) < (uint64_t)ncells) ++bits;
     return (bits + 7) / 8;
 }
 
-hipError_t launch_radix_sort(hipStream_t st, const SortPlan& sp) {
+hipError_t launch_sort_keys(hipStream_t st, const SortPlan& sp) {
     if (sp.n == 0) return hipSuccess;
+    // The cell-start table (both modes).
+    const uint32_t stiles = scan_tiles(sp.ncells);
+    hipLaunchKernelGGL(cell_scan_kernel, dim3(stiles), dim3(kSortBlock), 0, st, sp.count, sp.cstart, sp.ncells,
+                       sp.scan_status, sp.scan_status + stiles, sp.err);
+    if (sp.mode == kSortCellCount) {
+        hipLaunchKernelGGL(cell_scatter_kernel, dim3(blocks_for(sp.n, 256)), dim3(256), 0, st, sp.n, sp.ncells,
+                           sp.key0, sp.rank0, sp.cstart, sp.tmp, sp.err);
+        return hipGetLastError();
+    }
     const uint32_t tiles = sort_tiles(sp.n, sp.items);
     const uint32_t hist_blocks = (sp.n + kSortBlock * kHistItems - 1) / (kSortBlock * kHistItems);
     const uint32_t status_words = (uint32_t)sp.passes * tiles * kBins;
-    hipLaunchKernelGGL(radix_histogram_kernel, dim3(hist_blocks), dim3(kSortBlock), 0, st,
-                       sp.keys_in, sp.n, sp.passes, sp.hist, sp.status, status_words, sp.tickets,
-                       sp.prev_sorted, sp.prev_n, sp.cells);
+    hipLaunchKernelGGL(radix_histogram_kernel, dim3(hist_blocks), dim3(kSortBlock), 0, st, sp.key0, sp.n,
+                       sp.passes, sp.hist, sp.status, status_words, sp.tickets);
     for (int p = 0; p < sp.passes; ++p) {
-        const uint32_t* kin = p == 0 ? sp.keys_in : sp.kbuf[(p - 1) & 1];
+        const bool last = p == sp.passes - 1;
+        const uint32_t* kin = p == 0 ? sp.key0 : sp.kbuf[(p - 1) & 1];
         const uint32_t* vin = p == 0 ? nullptr : sp.vbuf[(p - 1) & 1];
-        uint32_t* kout = sp.kbuf[p & 1];
-        uint32_t* vout = sp.vbuf[p & 1];
+        uint32_t* kout = last ? sp.skey : sp.kbuf[p & 1];
+        uint32_t* vout = last ? sp.perm : sp.vbuf[p & 1];
         uint32_t* status = sp.status + (size_t)p * tiles * kBins;
         const uint32_t* hist = sp.hist + p * kBins;
         switch (sp.items) {
@@ -272,6 +429,21 @@ hipError_t launch_radix_sort(hipStream_t st, const SortPlan& sp) {
                 break;
         }
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_sort_gather(hipStream_t st, const SortPlan& sp) {
+    if (sp.n == 0) return hipSuccess;
+    const uint32_t scan_reset_words = scan_tiles(sp.ncells) + 1;  // status words + ticket
+    if (sp.mode == kSortCellCount)
+        hipLaunchKernelGGL(cell_rank_kernel, dim3(blocks_for(sp.n, sp.reorder_block)), dim3(sp.reorder_block), 0,
+                           st, sp.n, sp.ncells, sp.key0, sp.cstart, sp.tmp, sp.pos_in, sp.vel_in, sp.id_in,
+                           sp.pos_out, sp.vel_out, sp.id_out, sp.skey, sp.perm, sp.scan_status,
+                           scan_reset_words, sp.err);
+    else
+        hipLaunchKernelGGL(radix_reorder_kernel, dim3(blocks_for(sp.n, sp.reorder_block)), dim3(sp.reorder_block),
+                           0, st, sp.n, sp.perm, sp.pos_in, sp.vel_in, sp.id_in, sp.pos_out, sp.vel_out,
+                           sp.id_out, sp.hist, sp.scan_status, scan_reset_words, sp.err);
     return hipGetLastError();
 }
 

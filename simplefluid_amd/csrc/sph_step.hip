@@ -1,9 +1,9 @@
 // Per-particle passes of one SPH step on gfx950:
-//   hash     cell key of every state position (first step only; afterwards
-//            the integrate pass writes the next keys)
-//   reorder  gather state into sorted order, write cell start/end
+//   hash     cell key and per-cell count of every state position (first step
+//            only; afterwards the integrate pass produces the next keys)
 //   density  rho, EOS, neighbour bit masks (candidate tests happen here once)
 //   force    pressure + viscosity from the masks, integrate, clamp, next key
+// (sorting and the gather into sorted order live in sph_sort.hip).
 // Neighbour order is canonical: rows (dz, dy) = (-1..1, -1..1), dz outer; a
 // row is the contiguous sorted range of cells cx-1..cx+1; ascending index.
 // Density sums and force sums run in that order one particle per lane.
@@ -33,16 +33,23 @@ __device__ __forceinline__ void density_one(const StepArgs& a, const SceneConst&
     for (int row = 0; row < kRows; ++row) {
         const int dz = row / 3 - 1, dy = row % 3 - 1;
         uint32_t lo, hi;
-        row_range(a.cells, sc, cx, cy + dy, cz + dz, lo, hi);
+        row_range(a.cstart, sc, cx, cy + dy, cz + dz, lo, hi);
         uint32_t m = 0u;
-        for (uint32_t j = lo; j < hi; ++j) {
-            const float4 pj = a.pos_s[j];
-            const float d2 = dist2(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z);
-            const bool hit = d2 < h2;
-            const float t = hit ? h2 - d2 : 0.0f;
-            sum = fmaf(t * t, t, sum);
-            const uint32_t b = j - lo;
-            if (hit && b < (uint32_t)kMaskBits) m |= 1u << b;
+        // Four candidates per trip: their loads are in flight together (the
+        // tail re-reads the last candidate and masks it out).
+        for (uint32_t j = lo; j < hi; j += 4) {
+            float4 pj[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) pj[u] = a.pos_s[min(j + (uint32_t)u, hi - 1u)];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const float d2 = dist2(pi.x - pj[u].x, pi.y - pj[u].y, pi.z - pj[u].z);
+                const bool hit = (j + (uint32_t)u < hi) && d2 < h2;
+                const float t = hit ? h2 - d2 : 0.0f;
+                sum = fmaf(t * t, t, sum);
+                const uint32_t b = j + (uint32_t)u - lo;
+                if (hit && b < (uint32_t)kMaskBits) m |= 1u << b;
+            }
         }
         a.mask[(size_t)row * a.n + k] = m;
     }
@@ -57,14 +64,13 @@ struct ForceAcc {
     float px, py, pz, vx, vy, vz;
 };
 
-__device__ __forceinline__ void force_pair(const StepArgs& a, const SceneConst& sc, const float4& pi,
-                                           const float4& vi, float pti, uint32_t j, ForceAcc& f) {
-    const float4 pj = a.pos_s[j];
-    const float4 vj = a.vel_s[j];
-    const float2 xj = a.aux[j];
+__device__ __forceinline__ void force_pair(const SceneConst& sc, const float4& pi, const float4& vi, float pti,
+                                           const float4& pj, const float4& vj, const float2& xj, ForceAcc& f) {
     const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
     const float d2 = dist2(dx, dy, dz);
-    const float inv_r = d2 > 0.0f ? rsqrtf(d2) : 0.0f;
+    // Hardware reciprocal square root: d2 of two distinct particles is never
+    // denormal, so the denormal rescaling of rsqrtf is not needed.
+    const float inv_r = d2 > 0.0f ? __builtin_amdgcn_rsqf(d2) : 0.0f;
     const float r = d2 * inv_r;
     const float hr = sc.h - r;
     const float w = hr * hr;
@@ -90,22 +96,43 @@ __device__ __forceinline__ void force_one(const StepArgs& a, const SceneConst& s
     for (int row = 0; row < kRows; ++row) {
         const int dz = row / 3 - 1, dy = row % 3 - 1;
         uint32_t lo, hi;
-        row_range(a.cells, sc, cx, cy + dy, cz + dz, lo, hi);
+        row_range(a.cstart, sc, cx, cy + dy, cz + dz, lo, hi);
         uint32_t m = a.mask[(size_t)row * a.n + k];
         if (row == 4) {
             const uint32_t b = k - lo;
             if (b < (uint32_t)kMaskBits) m &= ~(1u << b);
         }
-        while (m) {
-            const uint32_t b = (uint32_t)__builtin_ctz(m);
+        if (m) {
+            // Software-pipelined: the next neighbour's loads are issued
+            // before the current neighbour's arithmetic.
+            uint32_t j = lo + (uint32_t)__builtin_ctz(m);
             m &= m - 1u;
-            force_pair(a, sc, pi, vi, xi.x, lo + b, f);
+            float4 pj = a.pos_s[j], vj = a.vel_s[j];
+            float2 xj = a.aux[j];
+            while (true) {
+                const bool more = m != 0u;
+                float4 pn = pj, vn = vj;
+                float2 xn = xj;
+                if (more) {
+                    j = lo + (uint32_t)__builtin_ctz(m);
+                    m &= m - 1u;
+                    pn = a.pos_s[j];
+                    vn = a.vel_s[j];
+                    xn = a.aux[j];
+                }
+                force_pair(sc, pi, vi, xi.x, pj, vj, xj, f);
+                if (!more) break;
+                pj = pn;
+                vj = vn;
+                xj = xn;
+            }
         }
         if (hi > lo + (uint32_t)kMaskBits) {
             for (uint32_t j = lo + (uint32_t)kMaskBits; j < hi; ++j) {
                 if (j == k) continue;
                 const float4 pj = a.pos_s[j];
-                if (dist2(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z) < sc.h2) force_pair(a, sc, pi, vi, xi.x, j, f);
+                if (dist2(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z) < sc.h2)
+                    force_pair(sc, pi, vi, xi.x, pj, a.vel_s[j], a.aux[j], f);
             }
         }
     }
@@ -129,7 +156,9 @@ __device__ __forceinline__ void force_one(const StepArgs& a, const SceneConst& s
     a.pos_o[k] = make_float4(p[0], p[1], p[2], 0.0f);
     a.vel_o[k] = make_float4(v[0], v[1], v[2], 0.0f);
     a.id_o[k] = a.id_s[k];
-    a.key0[k] = cell_key(sc, p[0], p[1], p[2]);
+    const uint32_t key = cell_key(sc, p[0], p[1], p[2]);
+    a.key0[k] = key;
+    a.rank0[k] = atomicAdd(&a.count[key], 1u);
 }
 
 template <int BLOCK>
@@ -157,6 +186,7 @@ __global__ __launch_bounds__(BLOCK) void force_kernel(StepArgs a) {
 }
 
 __global__ void hash_kernel(const float4* __restrict__ pos, uint32_t* __restrict__ key0,
+                            uint32_t* __restrict__ rank0, uint32_t* __restrict__ count,
                             const SceneConst* __restrict__ scenes, const uint2* __restrict__ tiles,
                             uint32_t ntiles) {
     uint32_t t0, t1;
@@ -167,32 +197,11 @@ __global__ void hash_kernel(const float4* __restrict__ pos, uint32_t* __restrict
         const uint32_t k = ti.y + threadIdx.x;
         if (k < sc.p_end) {
             const float4 p = pos[k];
-            key0[k] = cell_key(sc, p.x, p.y, p.z);
+            const uint32_t key = cell_key(sc, p.x, p.y, p.z);
+            key0[k] = key;
+            rank0[k] = atomicAdd(&count[key], 1u);
         }
     }
-}
-
-__global__ void reorder_kernel(uint32_t n, const uint32_t* __restrict__ perm,
-                               const uint32_t* __restrict__ skey, const float4* __restrict__ pos_in,
-                               const float4* __restrict__ vel_in, const uint32_t* __restrict__ id_in,
-                               float4* __restrict__ pos_out, float4* __restrict__ vel_out,
-                               uint32_t* __restrict__ id_out, uint2* __restrict__ cells,
-                               uint32_t ncells, uint32_t* __restrict__ hist_zero, uint32_t* __restrict__ err) {
-    if (hist_zero && blockIdx.x == 0)
-        for (uint32_t i = threadIdx.x; i < (uint32_t)(kMaxPasses * 256); i += blockDim.x) hist_zero[i] = 0u;
-    const uint32_t k = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    const uint32_t s = perm[k];
-    const uint32_t key = skey[k];
-    if (s >= n || key >= ncells) {  // a broken sort raises the flag instead of faulting
-        atomicOr(err, 4u);
-        return;
-    }
-    pos_out[k] = pos_in[s];
-    vel_out[k] = vel_in[s];
-    id_out[k] = id_in[s];
-    if (k == 0 || skey[k - 1] != key) cells[key].x = k;
-    if (k + 1 == n || skey[k + 1] != key) cells[key].y = k + 1;
 }
 
 // Neighbour sets exactly as force_one iterates them (self included).
@@ -215,7 +224,7 @@ __global__ void neighbors_kernel(StepArgs a, const uint32_t* __restrict__ offset
         for (int row = 0; row < kRows; ++row) {
             const int dz = row / 3 - 1, dy = row % 3 - 1;
             uint32_t lo, hi;
-            row_range(a.cells, sc, cx, cy + dy, cz + dz, lo, hi);
+            row_range(a.cstart, sc, cx, cy + dy, cz + dz, lo, hi);
             uint32_t m = a.mask[(size_t)row * a.n + k];
             while (m) {
                 const uint32_t b = (uint32_t)__builtin_ctz(m);
@@ -239,11 +248,6 @@ __global__ void neighbors_kernel(StepArgs a, const uint32_t* __restrict__ offset
 __global__ void fill_u32_kernel(uint32_t* __restrict__ p, uint32_t v, size_t count) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x)
         p[i] = v;
-}
-
-__global__ void fill_cells_kernel(uint2* __restrict__ c, size_t count) {
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x)
-        c[i] = make_uint2(0xffffffffu, 0u);
 }
 
 __global__ void unpermute3_kernel(uint32_t n, const float4* __restrict__ src,
@@ -296,19 +300,10 @@ hipError_t launch_force(hipStream_t st, int block, uint32_t grid, const StepArgs
 }
 
 hipError_t launch_hash(hipStream_t st, int block, uint32_t grid, const float4* pos, uint32_t* key0,
-                       const SceneConst* scenes, const uint2* tiles, uint32_t ntiles) {
+                       uint32_t* rank0, uint32_t* count, const SceneConst* scenes, const uint2* tiles,
+                       uint32_t ntiles) {
     if (grid == 0) return hipSuccess;
-    hipLaunchKernelGGL(hash_kernel, dim3(grid), dim3(block), 0, st, pos, key0, scenes, tiles, ntiles);
-    return hipGetLastError();
-}
-
-hipError_t launch_reorder(hipStream_t st, int block, uint32_t n, const uint32_t* perm,
-                          const uint32_t* skey, const float4* pos_in, const float4* vel_in,
-                          const uint32_t* id_in, float4* pos_out, float4* vel_out, uint32_t* id_out,
-                          uint2* cells, uint32_t ncells, uint32_t* hist_zero, uint32_t* err) {
-    const uint32_t grid = blocks_for(n, block) > 0 ? blocks_for(n, block) : 1;
-    hipLaunchKernelGGL(reorder_kernel, dim3(grid), dim3(block), 0, st, n, perm, skey, pos_in, vel_in,
-                       id_in, pos_out, vel_out, id_out, cells, ncells, hist_zero, err);
+    hipLaunchKernelGGL(hash_kernel, dim3(grid), dim3(block), 0, st, pos, key0, rank0, count, scenes, tiles, ntiles);
     return hipGetLastError();
 }
 
@@ -326,13 +321,6 @@ hipError_t launch_fill_u32(hipStream_t st, uint32_t* p, uint32_t value, size_t c
     if (count == 0) return hipSuccess;
     const uint32_t g = blocks_for(count, 256) < 4096 ? blocks_for(count, 256) : 4096;
     hipLaunchKernelGGL(fill_u32_kernel, dim3(g), dim3(256), 0, st, p, value, count);
-    return hipGetLastError();
-}
-
-hipError_t launch_fill_cells(hipStream_t st, uint2* cells, size_t count) {
-    if (count == 0) return hipSuccess;
-    const uint32_t g = blocks_for(count, 256) < 4096 ? blocks_for(count, 256) : 4096;
-    hipLaunchKernelGGL(fill_cells_kernel, dim3(g), dim3(256), 0, st, cells, count);
     return hipGetLastError();
 }
 

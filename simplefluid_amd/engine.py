@@ -52,8 +52,11 @@ EXPORTS = (
     "sph_rollout_backward", "sph_profile_enable", "sph_profile_read", "sph_autotune",
     "sph_get_tuning", "sph_set_tuning", "sph_debug_constants", "sph_debug_get_state",
     "sph_debug_set_state", "sph_debug_keep_acc", "sph_debug_step_outputs",
-    "sph_debug_num_cells", "sph_debug_cells", "sph_debug_neighbors",
+    "sph_debug_num_cells", "sph_debug_cells", "sph_debug_neighbors", "sph_set_sort_mode",
+    "sph_get_sort_mode",
 )
+SORT_CELL_COUNT = 0
+SORT_RADIX = 1
 
 
 class SPHError(RuntimeError):
@@ -138,6 +141,8 @@ def load_library(path: str = LIB_PATH):
         "sph_debug_num_cells": (u32, [H]),
         "sph_debug_cells": (i32, [H, vp, vp]),
         "sph_debug_neighbors": (i32, [H, vp, vp, C.c_uint64, P(C.c_uint64)]),
+        "sph_set_sort_mode": (i32, [H, i32]),
+        "sph_get_sort_mode": (i32, [H]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -319,6 +324,12 @@ class Engine:
     def set_tuning(self, density_block=256, force_block=256, reorder_block=256, sort_items=8):
         cfg = np.array([density_block, force_block, reorder_block, sort_items], np.int32)
         self._chk(self._L.sph_set_tuning(self._h, _p(cfg)))
+
+    def set_sort_mode(self, mode: int):
+        self._chk(self._L.sph_set_sort_mode(self._h, int(mode)))
+
+    def get_sort_mode(self) -> int:
+        return int(self._L.sph_get_sort_mode(self._h))
 
     # -- rollout adjoint ------------------------------------------------------
     def rollout_forward(self, steps: int, checkpoint_every: int = 20):

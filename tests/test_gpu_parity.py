@@ -160,6 +160,25 @@ def test_runs_are_deterministic_and_tuning_invariant(E):
         assert np.array_equal(res[0], r)
 
 
+def test_sort_modes_agree(E):
+    """Cell-count sort and 8-bit radix sort: same permutation, same results."""
+    from simplefluid_amd import configs
+    cfg = configs.dambreak_65k(jitter=1e-6, seed=8)
+    out = []
+    for mode in (E.SORT_CELL_COUNT, E.SORT_RADIX):
+        e = make_engine(E, cfg["params"], cfg["pos"])
+        e.set_sort_mode(mode)
+        assert e.get_sort_mode() == mode
+        e.advance_steps(20)
+        if mode == E.SORT_RADIX:
+            step_and_compare(E, e, cfg["params"], "radix sort step")
+        else:
+            e.advance_steps(1)
+        out.append((e.get_particles(), e.debug_step_outputs()["perm"]))
+        e.close()
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
 def test_get_particles_original_order(E):
     from simplefluid_amd import configs
     cfg = configs.dambreak_65k()
