@@ -169,11 +169,16 @@ int  sph_set_tuning(sph_handle* h, const int* cfg4);
 #define SPH_SORT_RADIX      1
 int  sph_set_sort_mode(sph_handle* h, int mode);
 int  sph_get_sort_mode(const sph_handle* h);
+/* Cells are cs = h(1 + 2^-10) in y and z and cs/xsub in x (no reference
+ * counterpart).  xsub = 0 picks automatically (4, 2 or 1 by cell budget);
+ * 1, 2, 4 force it.  Finer x cells let each neighbour row skip the x cells
+ * outside the support sphere's chord.                                       */
+int  sph_set_cell_subdivision(sph_handle* h, int xsub);
 
 /* ---- test-only views (parity against the CPU oracle) ---- */
 /* Derived constants of scene s: h, h2, inv_cs, mass, cD, cP, cV, inv_rho0,
- * lo[3], hi[3] (14 floats) and the grid G[3].                               */
-int  sph_debug_constants(const sph_handle* h, uint32_t scene, float* out14, int* G3);
+ * lo[3], hi[3] (14 floats) and the grid G[3] plus the x subdivision (4 ints). */
+int  sph_debug_constants(const sph_handle* h, uint32_t scene, float* out14, int* G4);
 /* State in the engine's internal order (the sorted order of the last step):
  * positions and velocities as 3n floats, original ids as n uint32.          */
 int  sph_debug_get_state(sph_handle* h, float* xyz, float* vel, uint32_t* ids);

@@ -34,6 +34,7 @@ class Params(C.Structure):
         ("dt", C.c_float),
         ("box_min", C.c_float * 3),
         ("box_max", C.c_float * 3),
+        ("x_subdivision", C.c_int),
     ]
 
 
@@ -43,7 +44,7 @@ def make_params(**kw) -> Params:
              pressure_stiffness=50.0, viscosity=0.01, boundary_restitution=0.1,
              use_attractive_pressure=0, attractive_pressure_ratio=0.1,
              gravity=(0.0, -9.81, 0.0), dt=1e-4, box_min=(0.0, 0.0, 0.0),
-             box_max=(1.0, 1.0, 1.0))
+             box_max=(1.0, 1.0, 1.0), x_subdivision=1)
     d.update(kw)
     for k, v in d.items():
         if k in ("gravity", "box_min", "box_max"):
@@ -53,14 +54,16 @@ def make_params(**kw) -> Params:
     return p
 
 
-def from_params(p) -> Params:
+def from_params(p, x_subdivision: int = 1) -> Params:
     """Oracle params from any object with the same field names (e.g. the
-    engine's sph_params)."""
+    engine's sph_params) plus the grid's x subdivision."""
     kw = {}
     for f, _ in Params._fields_:
+        if f == "x_subdivision":
+            continue
         v = getattr(p, f)
         kw[f] = list(v) if f in ("gravity", "box_min", "box_max") else v
-    return make_params(**kw)
+    return make_params(x_subdivision=x_subdivision, **kw)
 
 
 def build(force: bool = False) -> str:
@@ -163,11 +166,12 @@ class Oracle:
         return P, V
 
     def constants(self):
-        out = np.empty(14); G = np.empty(3, np.int32)
+        out = np.empty(14); G = np.empty(4, np.int32)
         self._L.orc_get_constants(self._h, _ptr(out), _ptr(G))
         names = ["h", "h2", "inv_cs", "mass", "cD", "cP", "cV", "inv_rho0"]
         d = {k: out[i] for i, k in enumerate(names)}
-        d["lo"] = out[8:11].copy(); d["hi"] = out[11:14].copy(); d["G"] = G
+        d["lo"] = out[8:11].copy(); d["hi"] = out[11:14].copy(); d["G"] = G[:3].copy()
+        d["xsub"] = int(G[3])
         return d
 
     def step_debug(self):

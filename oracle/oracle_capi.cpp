@@ -106,7 +106,7 @@ void orc_get_state(void* hp, double* pos3, double* vel3, uint32_t* ids) {
 }
 
 // Derived constants: h, h2, inv_cs, mass, cD, cP, cV, inv_rho0, lo[3], hi[3] (14 doubles)
-// and grid G[3] (3 ints).
+// and grid G[3] plus the x subdivision (4 ints).
 void orc_get_constants(void* hp, double* out14, int* G3) {
     Handle* h = (Handle*)hp;
     auto put = [&](auto& c) {
@@ -116,6 +116,7 @@ void orc_get_constants(void* hp, double* out14, int* G3) {
                         (double)c.hi[0], (double)c.hi[1], (double)c.hi[2]};
         for (int i = 0; i < 14; ++i) out14[i] = v[i];
         for (int i = 0; i < 3; ++i) G3[i] = c.G[i];
+        G3[3] = c.xsub;
     };
     if (h->fp64) put(h->d.c); else put(h->f.c);
 }
@@ -130,13 +131,15 @@ void orc_get_step_debug(void* hp, uint32_t* key, uint32_t* perm, uint32_t* skey,
     else get_dbg_t(h->f, key, perm, skey, rho, acc3, sid);
 }
 
-// Dense cell table of the last step (ncells entries each), empty = (~0, 0).
+// Cell table of the last step as (start, end) per cell, empty = (~0, 0).
 void orc_get_cells(void* hp, uint32_t* start, uint32_t* end) {
     Handle* h = (Handle*)hp;
     auto& cs = h->fp64 ? h->d.cstart : h->f.cstart;
-    auto& ce = h->fp64 ? h->d.cend : h->f.cend;
-    std::copy(cs.begin(), cs.end(), start);
-    std::copy(ce.begin(), ce.end(), end);
+    for (size_t i = 0; i + 1 < cs.size(); ++i) {
+        const bool empty = cs[i] == cs[i + 1];
+        start[i] = empty ? ~0u : cs[i];
+        end[i] = empty ? 0u : cs[i + 1];
+    }
 }
 
 uint64_t orc_num_neighbor_entries(void* hp) {

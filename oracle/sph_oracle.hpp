@@ -21,8 +21,12 @@
 // Every op sequence that decides a discrete outcome (cell index, neighbour
 // predicate) is written exactly as the engine writes it, so cells and
 // neighbour sets are comparable bit for bit.  Float sums run in the canonical
-// neighbour order: rows dz = -1..1 (outer), dy = -1..1 (inner), each row the
-// contiguous sorted range of cells cx-1..cx+1, ascending sorted index.
+// neighbour order: rows dz = -1..1 (outer), dy = -1..1 (inner) of y/z cells,
+// each row a contiguous sorted range of x cells, ascending sorted index.
+// Cells are cs = h(1 + 2^-10) in y and z and cs/xsub in x; a row here spans
+// the x cells within xsub of the particle's own (a superset of its
+// neighbours, filtered by the predicate), so the set and the order of the
+// neighbours match the engine's narrower per-row windows exactly.
 #pragma once
 
 #include <algorithm>
@@ -51,11 +55,13 @@ struct Params {
     float dt;
     float box_min[3];
     float box_max[3];
+    int   x_subdivision;   // x cells per y/z cell (1, 2 or 4)
 };
 
 template <class T>
 struct Consts {
-    T h, h2, inv_cs, mass, cD, cP, cV, inv_rho0, B, e, attr, dt, r;
+    T h, h2, inv_cs, inv_csx, mass, cD, cP, cV, inv_rho0, B, e, attr, dt, r;
+    int xsub;
     T g[3], bmin[3], lo[3], hi[3];
     int G[3];
     bool attractive;
@@ -84,6 +90,8 @@ inline Consts<T> derive(const Params& p) {
     c.cV        = (T)((double)(T)p.viscosity * (double)c.mass * spiky);
     c.inv_rho0  = (T)(1.0 / (double)(T)p.rest_density);
     c.inv_cs    = (T)(1.0 / (hd * (1.0 + 1.0 / 1024.0)));
+    c.xsub      = p.x_subdivision > 0 ? p.x_subdivision : 1;
+    c.inv_csx   = c.inv_cs * (T)c.xsub;
     c.B         = (T)p.pressure_stiffness;
     c.e         = (T)p.boundary_restitution;
     c.attr      = (T)p.attractive_pressure_ratio;
@@ -94,7 +102,7 @@ inline Consts<T> derive(const Params& p) {
         c.g[k]    = (T)p.gravity[k];
         c.bmin[k] = (T)p.box_min[k];
         const T ext = (T)p.box_max[k] - (T)p.box_min[k];
-        int G = (int)std::ceil(ext * c.inv_cs);
+        int G = (int)std::ceil(ext * (k == 0 ? c.inv_csx : c.inv_cs));
         c.G[k] = G < 1 ? 1 : G;
         c.lo[k] = (T)p.box_min[k] + r;
         c.hi[k] = (T)p.box_max[k] - r;
@@ -112,7 +120,7 @@ inline int cell_coord(T x, T bmin, T inv_cs, int G) {
 
 template <class T>
 inline uint32_t cell_key(const Consts<T>& c, T x, T y, T z) {
-    const int cx = cell_coord(x, c.bmin[0], c.inv_cs, c.G[0]);
+    const int cx = cell_coord(x, c.bmin[0], c.inv_csx, c.G[0]);
     const int cy = cell_coord(y, c.bmin[1], c.inv_cs, c.G[1]);
     const int cz = cell_coord(z, c.bmin[2], c.inv_cs, c.G[2]);
     return ((uint32_t)cz * (uint32_t)c.G[1] + (uint32_t)cy) * (uint32_t)c.G[0] + (uint32_t)cx;
@@ -147,7 +155,7 @@ struct Sim {
     std::vector<uint32_t> key;         // state order, key of the input state
     std::vector<uint32_t> perm;        // sorted index -> state index
     std::vector<uint32_t> skey;        // sorted keys
-    std::vector<uint32_t> cstart, cend; // dense cell table, empty = (~0u, 0)
+    std::vector<uint32_t> cstart;      // C + 1: cstart[c] = number of particles with key < c
     std::vector<T> sx, sy, sz, svx, svy, svz;  // reordered input state
     std::vector<T> rho, pterm, irho, ax, ay, az; // sorted order
     std::vector<uint32_t> noff, nbr;   // neighbour CSR in sorted index space
@@ -225,14 +233,18 @@ struct Sim {
     }
 
     void row_range(int cx, int y_, int z_, uint32_t& lo, uint32_t& hi) const {
-        lo = ~0u; hi = 0;
+        lo = 0; hi = 0;
         if (y_ < 0 || y_ >= c.G[1] || z_ < 0 || z_ >= c.G[2]) return;
-        const int x0 = std::max(cx - 1, 0), x1 = std::min(cx + 1, c.G[0] - 1);
+        const int x0 = std::max(cx - c.xsub, 0), x1 = std::min(cx + c.xsub, c.G[0] - 1);
         const uint32_t base = ((uint32_t)z_ * (uint32_t)c.G[1] + (uint32_t)y_) * (uint32_t)c.G[0];
-        for (int xx = x0; xx <= x1; ++xx) {
-            lo = std::min(lo, cstart[base + xx]);
-            hi = std::max(hi, cend[base + xx]);
-        }
+        lo = cstart[base + x0];
+        hi = cstart[base + x1 + 1];
+    }
+
+    void coords(T x_, T y_, T z_, int& cx, int& cy, int& cz) const {
+        cx = cell_coord(x_, c.bmin[0], c.inv_csx, c.G[0]);
+        cy = cell_coord(y_, c.bmin[1], c.inv_cs, c.G[1]);
+        cz = cell_coord(z_, c.bmin[2], c.inv_cs, c.G[2]);
     }
 
     void step() {
@@ -252,22 +264,18 @@ struct Sim {
             svx[k] = vx[s]; svy[k] = vy[s]; svz[k] = vz[s];
             sid[k] = id[s];
         }
-        // Cell table.
-        cstart.assign(c.ncells, ~0u);
-        cend.assign(c.ncells, 0u);
-        for (uint32_t k = 0; k < n; ++k) {
-            if (k == 0 || skey[k - 1] != skey[k]) cstart[skey[k]] = k;
-            if (k + 1 == n || skey[k + 1] != skey[k]) cend[skey[k]] = k + 1;
-        }
+        // Cell-start table: counts, exclusive prefix sum.
+        cstart.assign((size_t)c.ncells + 1, 0u);
+        for (uint32_t k = 0; k < n; ++k) cstart[skey[k] + 1]++;
+        for (uint32_t q = 0; q < c.ncells; ++q) cstart[q + 1] += cstart[q];
         // Density + EOS.
         rho.resize(n); pterm.resize(n); irho.resize(n);
         const T h2 = c.h2;
 #pragma omp parallel for num_threads(nt) schedule(static)
         for (long long k = 0; k < (long long)n; ++k) {
             const T xi = sx[k], yi = sy[k], zi = sz[k];
-            const int cx = cell_coord(xi, c.bmin[0], c.inv_cs, c.G[0]);
-            const int cy = cell_coord(yi, c.bmin[1], c.inv_cs, c.G[1]);
-            const int cz = cell_coord(zi, c.bmin[2], c.inv_cs, c.G[2]);
+            int cx, cy, cz;
+            coords(xi, yi, zi, cx, cy, cz);
             T sum = 0;
             for (int dz = -1; dz <= 1; ++dz)
                 for (int dy = -1; dy <= 1; ++dy) {
@@ -295,9 +303,8 @@ struct Sim {
 #pragma omp parallel for num_threads(nt) schedule(static)
             for (long long k = 0; k < (long long)n; ++k) {
                 const T xi = sx[k], yi = sy[k], zi = sz[k];
-                const int cx = cell_coord(xi, c.bmin[0], c.inv_cs, c.G[0]);
-                const int cy = cell_coord(yi, c.bmin[1], c.inv_cs, c.G[1]);
-                const int cz = cell_coord(zi, c.bmin[2], c.inv_cs, c.G[2]);
+                int cx, cy, cz;
+                coords(xi, yi, zi, cx, cy, cz);
                 for (int dz = -1; dz <= 1; ++dz)
                     for (int dy = -1; dy <= 1; ++dy) {
                         uint32_t lo, hi;
@@ -318,9 +325,8 @@ struct Sim {
             const T xi = sx[k], yi = sy[k], zi = sz[k];
             const T vxi = svx[k], vyi = svy[k], vzi = svz[k];
             const T pti = pterm[k];
-            const int cx = cell_coord(xi, c.bmin[0], c.inv_cs, c.G[0]);
-            const int cy = cell_coord(yi, c.bmin[1], c.inv_cs, c.G[1]);
-            const int cz = cell_coord(zi, c.bmin[2], c.inv_cs, c.G[2]);
+            int cx, cy, cz;
+            coords(xi, yi, zi, cx, cy, cz);
             T apx = 0, apy = 0, apz = 0, avx = 0, avy = 0, avz = 0;
             for (int dz = -1; dz <= 1; ++dz)
                 for (int dy = -1; dy <= 1; ++dy) {

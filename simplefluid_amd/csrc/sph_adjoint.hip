@@ -35,14 +35,11 @@ __device__ __forceinline__ void tile_span(uint32_t ntiles, uint32_t& t0, uint32_
 template <class F>
 __device__ __forceinline__ void for_each_neighbor(const AdjArgs& a, const SceneConst& sc, uint32_t k,
                                                   const float4& pi, F&& f) {
-    const int cx = cell_coord(pi.x, sc.bmin[0], sc.inv_cs, sc.G[0]);
-    const int cy = cell_coord(pi.y, sc.bmin[1], sc.inv_cs, sc.G[1]);
-    const int cz = cell_coord(pi.z, sc.bmin[2], sc.inv_cs, sc.G[2]);
+    const RowGeom g = row_geom(sc, pi.x, pi.y, pi.z);
 #pragma unroll
     for (int row = 0; row < kRows; ++row) {
-        const int dz = row / 3 - 1, dy = row % 3 - 1;
         uint32_t lo, hi;
-        row_range(a.cstart, sc, cx, cy + dy, cz + dz, lo, hi);
+        row_window(a.cstart, sc, g, row % 3 - 1, row / 3 - 1, lo, hi);
         uint32_t m = a.mask[(size_t)row * a.n + k];
         if (row == 4) {
             const uint32_t b = k - lo;

@@ -43,9 +43,13 @@ struct SceneConst {
     uint32_t p_begin;  // particle range of this scene (state and sorted order)
     uint32_t p_end;
     float mu;
+    float inv_csx;     // x cells are cs/xsub wide: xsub * inv_cs
+    float xsubf;       // x cells per cs (1, 2 or 4)
+    float hc2;         // (h * inv_cs)^2: the support radius squared in y/z cell units
+    float wmargin;     // x-window safety margin in x cells
     float pad;
 };
-static_assert(sizeof(SceneConst) == 128, "SceneConst layout");
+static_assert(sizeof(SceneConst) == 144, "SceneConst layout");
 
 constexpr int kRows = 9;          // 3x3 (dz, dy) rows of three x-adjacent cells
 constexpr int kMaskBits = 32;     // candidates per row tracked in the mask
@@ -69,27 +73,100 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t bid, uint32_t nblocks) {
     return x * q + (x < r ? x : r) + i;
 }
 
-// Sorted-index range covering cells cx-1..cx+1 of grid row (y, z).  Cells of
-// a row are contiguous in sorted order and cstart holds a lower bound for
-// every cell (empty ones included), so the row is [cstart[first],
-// cstart[last + 1]).  Rows outside the grid are empty (lo = hi = 0).
-__device__ __forceinline__ void row_range(const uint32_t* __restrict__ cstart, const SceneConst& sc,
-                                          int cx, int y, int z, uint32_t& lo, uint32_t& hi) {
+// Neighbour rows.  Cells are cs = h(1 + 2^-10) in y and z and cs/xsub in x.
+// The candidates of a particle lie in the 3x3 rows (dy, dz) of y/z cells
+// around its own; within a row, only x cells inside the chord of the support
+// sphere at that row's nearest y/z distance can hold neighbours.  A row's x
+// cells are contiguous in sorted order, and cstart holds a lower bound for
+// every cell (empty ones included), so a row is the single range
+// [cstart[first x cell], cstart[last x cell + 1]).  The window is a superset
+// of the neighbours (distances are under-estimated by 2^-10 cells and the
+// chord is widened by wmargin); the predicate d^2 < h^2 decides membership.
+struct RowGeom {
+    float fx;          // x coordinate in x cells
+    int cy, cz;
+    float qy[3], qz[3];  // squared y / z distance to rows dy, dz = -1, 0, +1 (cell units)
+};
+
+__device__ __forceinline__ RowGeom row_geom(const SceneConst& sc, float x, float y, float z) {
+    RowGeom g;
+    g.fx = (x - sc.bmin[0]) * sc.inv_csx;
+    const float fy = (y - sc.bmin[1]) * sc.inv_cs;
+    const float fz = (z - sc.bmin[2]) * sc.inv_cs;
+    g.cy = (int)fminf(fmaxf(fy, 0.0f), (float)(sc.G[1] - 1));
+    g.cz = (int)fminf(fmaxf(fz, 0.0f), (float)(sc.G[2] - 1));
+    const float m = 0.0009765625f;
+    const float ry = fminf(fmaxf(fy - (float)g.cy, 0.0f), 1.0f);
+    const float rz = fminf(fmaxf(fz - (float)g.cz, 0.0f), 1.0f);
+    const float ly = fmaxf(ry - m, 0.0f), uy = fmaxf(1.0f - ry - m, 0.0f);
+    const float lz = fmaxf(rz - m, 0.0f), uz = fmaxf(1.0f - rz - m, 0.0f);
+    g.qy[0] = ly * ly; g.qy[1] = 0.0f; g.qy[2] = uy * uy;
+    g.qz[0] = lz * lz; g.qz[1] = 0.0f; g.qz[2] = uz * uz;
+    return g;
+}
+
+__device__ __forceinline__ void row_window(const uint32_t* __restrict__ cstart, const SceneConst& sc,
+                                           const RowGeom& g, int dy, int dz, uint32_t& lo, uint32_t& hi) {
     lo = 0u;
     hi = 0u;
+    const int y = g.cy + dy, z = g.cz + dz;
     if (y < 0 || y >= sc.G[1] || z < 0 || z >= sc.G[2]) return;
-    const int x0 = cx > 0 ? cx - 1 : 0;
-    const int x1 = cx + 1 < sc.G[0] ? cx + 1 : sc.G[0] - 1;
+    const float w2 = sc.hc2 - (g.qy[dy + 1] + g.qz[dz + 1]);
+    if (!(w2 > 0.0f)) return;
+    const float wx = __builtin_amdgcn_sqrtf(w2) * sc.xsubf + sc.wmargin;
+    const float gmax = (float)(sc.G[0] - 1);
+    const int x0 = (int)fminf(fmaxf(g.fx - wx, 0.0f), gmax);
+    const int x1 = (int)fminf(fmaxf(g.fx + wx, 0.0f), gmax);
     const uint32_t base = sc.cell_offset + ((uint32_t)z * (uint32_t)sc.G[1] + (uint32_t)y) * (uint32_t)sc.G[0];
     lo = cstart[base + (uint32_t)x0];
     hi = cstart[base + (uint32_t)x1 + 1u];
 }
 
+// Raw buffer resource over n elements of T: loads past the end return 0.
+template <class T>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_of(const T* p, uint32_t n) {
+    return __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, (int)(n * (uint32_t)sizeof(T)), 0x00020000);
+}
+
+__device__ __forceinline__ float4 load4(__amdgpu_buffer_rsrc_t r, uint32_t byte_off) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)byte_off, 0, 0);
+    return make_float4(__builtin_bit_cast(float, v[0]), __builtin_bit_cast(float, v[1]),
+                       __builtin_bit_cast(float, v[2]), __builtin_bit_cast(float, v[3]));
+}
+
+__device__ __forceinline__ float2 load2(__amdgpu_buffer_rsrc_t r, uint32_t byte_off) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, (int)byte_off, 0, 0);
+    return make_float2(__builtin_bit_cast(float, v[0]), __builtin_bit_cast(float, v[1]));
+}
+
 __device__ __forceinline__ uint32_t cell_key(const SceneConst& sc, float x, float y, float z) {
-    const int cx = cell_coord(x, sc.bmin[0], sc.inv_cs, sc.G[0]);
+    const int cx = cell_coord(x, sc.bmin[0], sc.inv_csx, sc.G[0]);
     const int cy = cell_coord(y, sc.bmin[1], sc.inv_cs, sc.G[1]);
     const int cz = cell_coord(z, sc.bmin[2], sc.inv_cs, sc.G[2]);
     return sc.cell_offset + ((uint32_t)cz * (uint32_t)sc.G[1] + (uint32_t)cy) * (uint32_t)sc.G[0] + (uint32_t)cx;
+}
+
+// Per-cell count with one atomic per run of equal keys in consecutive lanes
+// (lanes follow sorted order, so most neighbours share a cell).  Returns the
+// slot this lane took in its cell.  All lanes of the wave must call it;
+// `active` marks the lanes that hold a particle.
+__device__ __forceinline__ uint32_t count_slot(uint32_t* __restrict__ count, uint32_t key, bool active) {
+    const int lane = (int)(threadIdx.x & 63u);
+    const uint32_t kk = active ? key : 0xffffffffu;
+    const uint32_t prev = __shfl_up(kk, 1, 64);
+    const bool start = active && (lane == 0 || prev != kk);
+    const uint64_t starts = __ballot(start);
+    const uint64_t act = __ballot(active);
+    // Leader of this lane's run: the highest run start at or below the lane.
+    const uint64_t le = starts & (lane == 63 ? ~0ull : ((2ull << lane) - 1ull));
+    const int leader = le ? 63 - __clzll(le) : lane;
+    // Run length: up to the next start or the first inactive lane above.
+    const uint64_t above = (starts | ~act) & ~((2ull << lane) - 1ull) & (lane == 63 ? 0ull : ~0ull);
+    const int next = above ? __ffsll((long long)above) - 1 : 64;
+    uint32_t base = 0;
+    if (start) base = atomicAdd(&count[kk], (uint32_t)(next - lane));
+    base = __shfl(base, leader, 64);
+    return base + (uint32_t)(lane - leader);
 }
 
 // Tait EOS, exponent 7 by the chain x^2, x^4, (x^4 x^2) x; p < 0 is clamped

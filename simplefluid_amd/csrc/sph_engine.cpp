@@ -32,7 +32,7 @@ static bool finite3(const float* v) { return std::isfinite(v[0]) && std::isfinit
 
 // Derived constants, formed exactly as the CPU restatement forms them:
 // normalisations in double from the float inputs, rounded once to float.
-int derive_scene(const sph_params& p, SceneConst& sc, std::string& err) {
+int derive_scene(const sph_params& p, SceneConst& sc, std::string& err, int xsub) {
     if (!(p.particle_radius > 0.f) || !std::isfinite(p.particle_radius)) { err = "particle_radius must be > 0"; return SPH_E_INVALID; }
     if (!(p.kernel_radius > 0.f) || !std::isfinite(p.kernel_radius)) { err = "kernel_radius must be > 0"; return SPH_E_INVALID; }
     if (!(p.rest_density > 0.f) || !std::isfinite(p.rest_density)) { err = "rest_density must be > 0"; return SPH_E_INVALID; }
@@ -57,6 +57,14 @@ int derive_scene(const sph_params& p, SceneConst& sc, std::string& err) {
     sc.cV = (float)((double)p.viscosity * (double)mass * spiky);
     sc.inv_rho0 = (float)(1.0 / (double)p.rest_density);
     sc.inv_cs = (float)(1.0 / (hd * (1.0 + 1.0 / 1024.0)));
+    if (xsub != 1 && xsub != 2 && xsub != 4) { err = "x subdivision must be 1, 2 or 4"; return SPH_E_INVALID; }
+    sc.inv_csx = sc.inv_cs * (float)xsub;  // exact: power-of-two scale
+    sc.xsubf = (float)xsub;
+    {
+        const float hh = sc.h * sc.inv_cs;
+        sc.hc2 = hh * hh;
+    }
+    sc.wmargin = 1.0f / 256.0f;
     sc.B = p.pressure_stiffness;
     sc.attr = p.use_attractive_pressure ? p.attractive_pressure_ratio : 0.0f;
     sc.attractive = p.use_attractive_pressure ? 1 : 0;
@@ -69,9 +77,12 @@ int derive_scene(const sph_params& p, SceneConst& sc, std::string& err) {
         sc.bmin[k] = p.box_min[k];
         const float ext = p.box_max[k] - p.box_min[k];
         if (!(ext > 2.0f * r)) { err = "box must be wider than 2*particle_radius on every axis"; return SPH_E_INVALID; }
-        int G = (int)std::ceil(ext * sc.inv_cs);
+        int G = (int)std::ceil(ext * (k == 0 ? sc.inv_csx : sc.inv_cs));
         if (G < 1) G = 1;
-        if (G > 1024) { err = "grid exceeds 1024 cells on an axis (box / kernel_radius too large)"; return SPH_E_INVALID; }
+        if (G > (k == 0 ? 1024 * xsub : 1024)) {
+            err = "grid exceeds 1024 cells on an axis (box / kernel_radius too large)";
+            return SPH_E_INVALID;
+        }
         sc.G[k] = G;
         sc.lo[k] = p.box_min[k] + r;
         sc.hi[k] = p.box_max[k] - r;
@@ -287,13 +298,33 @@ int sync(sph_handle* h) {
 // C ABI
 // ---------------------------------------------------------------------------
 
+// x subdivision of the cells: the largest of 4, 2, 1 whose cell count stays
+// within max(4N, 2^20), so the per-step cell scan stays small next to the
+// particle traffic; a caller override (sph_set_cell_subdivision) wins.
+static int pick_xsub(const std::vector<sph_params>& ps, uint64_t n, int override_xsub) {
+    if (override_xsub) return override_xsub;
+    uint64_t cells1 = 0;
+    for (const sph_params& p : ps) {
+        SceneConst sc;
+        std::string err;
+        if (derive_scene(p, sc, err, 1) != SPH_OK) return 1;
+        cells1 += (uint64_t)sc.G[0] * sc.G[1] * sc.G[2];
+    }
+    const uint64_t budget = std::max<uint64_t>(4 * n, 1u << 20);
+    for (int x : {4, 2}) if (cells1 * (uint64_t)x <= budget) return x;
+    return 1;
+}
+
 static int configure(sph_handle* h, const std::vector<sph_params>& ps, const std::vector<uint32_t>& ns) {
     std::vector<SceneConst> sc(ps.size());
     uint64_t cell_off = 0, p_off = 0;
     std::vector<uint32_t> off(ps.size() + 1, 0);
+    uint64_t ntot = 0;
+    for (uint32_t v : ns) ntot += v;
+    const int xsub = pick_xsub(ps, ntot, h->xsub_override);
     for (size_t s = 0; s < ps.size(); ++s) {
         std::string err;
-        const int rc = derive_scene(ps[s], sc[s], err);
+        const int rc = derive_scene(ps[s], sc[s], err, xsub);
         if (rc != SPH_OK) return fail(h, rc, ps.size() > 1 ? "scene " + std::to_string(s) + ": " + err : err);
         if (ps[s].dt != ps[0].dt) return fail(h, SPH_E_INVALID, "all scenes of a batch must share dt");
         sc[s].cell_offset = (uint32_t)cell_off;
@@ -304,7 +335,7 @@ static int configure(sph_handle* h, const std::vector<sph_params>& ps, const std
         off[s + 1] = (uint32_t)p_off;
     }
     if (cell_off > (1ull << 30)) return fail(h, SPH_E_INVALID, "too many grid cells in total");
-    if (p_off >= (1ull << 30)) return fail(h, SPH_E_INVALID, "too many particles (limit 2^30)");
+    if (p_off >= (1ull << 28)) return fail(h, SPH_E_INVALID, "too many particles (limit 2^28)");
     h->params = ps;
     h->scene_n = ns;
     h->scene_off = off;
@@ -312,6 +343,7 @@ static int configure(sph_handle* h, const std::vector<sph_params>& ps, const std
     h->n = (uint32_t)p_off;
     h->ncells = (uint32_t)cell_off;
     h->passes = sort_key_passes(h->ncells);
+    h->xsub = xsub;
     h->tiles.clear();
     h->ntiles.clear();
     invalidate_graph(h);
@@ -375,7 +407,7 @@ int sph_create(const sph_params* p, sph_handle** out) {
     if (rc != SPH_OK) return rc;
     SceneConst tmp;
     std::string err;
-    rc = derive_scene(*p, tmp, err);
+    rc = derive_scene(*p, tmp, err, 1);
     if (rc != SPH_OK) {
         delete h;
         return rc;
@@ -397,7 +429,7 @@ int sph_set_params(sph_handle* h, const sph_params* p) {
     if (h->batched) return fail(h, SPH_E_INVALID, "use sph_set_scene_params on a batch handle");
     SceneConst tmp;
     std::string err;
-    int rc = derive_scene(*p, tmp, err);
+    int rc = derive_scene(*p, tmp, err, 1);
     if (rc != SPH_OK) return fail(h, rc, err);
     h->params[0] = *p;
     if (h->has_particles) {
@@ -629,7 +661,7 @@ int sph_set_scene_params(sph_handle* h, uint32_t scene, const sph_params* p) {
     if (!same_geom) return fail(h, SPH_E_INVALID, "box, radii, rest density and dt are fixed at create time");
     SceneConst sc;
     std::string err;
-    int rc = derive_scene(*p, sc, err);
+    int rc = derive_scene(*p, sc, err, h->xsub);
     if (rc != SPH_OK) return fail(h, rc, err);
     sc.cell_offset = h->sc_h[scene].cell_offset;
     sc.p_begin = h->sc_h[scene].p_begin;
@@ -707,19 +739,31 @@ int sph_set_sort_mode(sph_handle* h, int mode) {
     return SPH_OK;
 }
 
-int sph_get_sort_mode(const sph_handle* h) { return h ? h->tune.sort_mode : SPH_E_INVALID;
+int sph_get_sort_mode(const sph_handle* h) { return h ? h->tune.sort_mode : SPH_E_INVALID; }
+
+int sph_set_cell_subdivision(sph_handle* h, int xsub) {
+    if (!h) return SPH_E_INVALID;
+    if (xsub != 0 && xsub != 1 && xsub != 2 && xsub != 4) return fail(h, SPH_E_INVALID, "x subdivision must be 0, 1, 2 or 4");
+    h->xsub_override = xsub;
+    if (!h->has_particles) return SPH_OK;
+    // Re-grid keeping the state (it stays in its current order; keys are rebuilt).
+    const int rc = configure(h, h->params, h->scene_n);
+    if (rc != SPH_OK) return rc;
+    h->keys_valid = false;
+    h->ready = false;
+    return sph_make_ready(h);
 }
 
 // ---- test-only views ----
 
-int sph_debug_constants(const sph_handle* h, uint32_t scene, float* out14, int* G3) {
-    if (!h || !out14 || !G3) return SPH_E_INVALID;
+int sph_debug_constants(const sph_handle* h, uint32_t scene, float* out14, int* G4) {
+    if (!h || !out14 || !G4) return SPH_E_INVALID;
     SceneConst sc;
     if (scene < h->sc_h.size()) {
         sc = h->sc_h[scene];
     } else if (scene == 0) {
         std::string err;
-        if (derive_scene(h->params[0], sc, err) != SPH_OK) return SPH_E_INVALID;
+        if (derive_scene(h->params[0], sc, err, 1) != SPH_OK) return SPH_E_INVALID;
     } else {
         return SPH_E_INVALID;
     }
@@ -729,7 +773,8 @@ int sph_debug_constants(const sph_handle* h, uint32_t scene, float* out14, int* 
     const float v[14] = {sc.h, sc.h2, sc.inv_cs, mass, sc.cD, sc.cP, sc.cV,
                          sc.inv_rho0, sc.lo[0], sc.lo[1], sc.lo[2], sc.hi[0], sc.hi[1], sc.hi[2]};
     std::copy(v, v + 14, out14);
-    for (int k = 0; k < 3; ++k) G3[k] = sc.G[k];
+    for (int k = 0; k < 3; ++k) G4[k] = sc.G[k];
+    G4[3] = (int)sc.xsubf;
     return SPH_OK;
 }
 

@@ -86,6 +86,8 @@ struct sph_handle {
     uint32_t n = 0;
     uint32_t ncells = 0;
     int passes = 1;
+    int xsub = 1;                          // x cells per y/z cell size
+    int xsub_override = 0;                 // 0 = automatic
     bool ready = false;
     bool has_particles = false;
     bool has_step = false;
@@ -129,7 +131,7 @@ namespace sph {
 
 int fail(sph_handle* h, int code, const std::string& msg);
 int hip_fail(sph_handle* h, hipError_t e, const char* what);
-int derive_scene(const sph_params& p, SceneConst& sc, std::string& err);
+int derive_scene(const sph_params& p, SceneConst& sc, std::string& err, int xsub);
 int alloc_step_bufs(sph_handle* h, StepBufs& w);
 void invalidate_graph(sph_handle* h);
 uint2* tiles_for(sph_handle* h, int block, uint32_t& ntiles);

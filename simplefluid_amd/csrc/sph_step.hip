@@ -22,32 +22,34 @@ __device__ __forceinline__ void tile_span(uint32_t ntiles, uint32_t& t0, uint32_
     t1 = t0 + per < ntiles ? t0 + per : ntiles;
 }
 
-__device__ __forceinline__ void density_one(const StepArgs& a, const SceneConst& sc, uint32_t k) {
+__device__ __forceinline__ void density_one(const StepArgs& a, const SceneConst& sc, uint32_t k,
+                                            __amdgpu_buffer_rsrc_t rpos) {
     const float4 pi = a.pos_s[k];
-    const int cx = cell_coord(pi.x, sc.bmin[0], sc.inv_cs, sc.G[0]);
-    const int cy = cell_coord(pi.y, sc.bmin[1], sc.inv_cs, sc.G[1]);
-    const int cz = cell_coord(pi.z, sc.bmin[2], sc.inv_cs, sc.G[2]);
+    const RowGeom g = row_geom(sc, pi.x, pi.y, pi.z);
+    uint32_t lo[kRows], hi[kRows];
+#pragma unroll
+    for (int row = 0; row < kRows; ++row) row_window(a.cstart, sc, g, row % 3 - 1, row / 3 - 1, lo[row], hi[row]);
     const float h2 = sc.h2;
     float sum = 0.0f;
 #pragma unroll
     for (int row = 0; row < kRows; ++row) {
-        const int dz = row / 3 - 1, dy = row % 3 - 1;
-        uint32_t lo, hi;
-        row_range(a.cstart, sc, cx, cy + dy, cz + dz, lo, hi);
         uint32_t m = 0u;
-        // Four candidates per trip: their loads are in flight together (the
-        // tail re-reads the last candidate and masks it out).
-        for (uint32_t j = lo; j < hi; j += 4) {
+        const uint32_t l = lo[row], e = hi[row];
+        // Four candidates per trip, loads in flight together; slots past the
+        // row end are masked (past the array end the buffer returns 0).
+        for (uint32_t j = l; j < e; j += 4) {
+            const uint32_t rem = e - j;
+            const uint32_t off = j << 4;
             float4 pj[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) pj[u] = a.pos_s[min(j + (uint32_t)u, hi - 1u)];
+            for (int u = 0; u < 4; ++u) pj[u] = load4(rpos, off + 16u * (uint32_t)u);
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const float d2 = dist2(pi.x - pj[u].x, pi.y - pj[u].y, pi.z - pj[u].z);
-                const bool hit = (j + (uint32_t)u < hi) && d2 < h2;
+                const bool hit = ((uint32_t)u < rem) && d2 < h2;
                 const float t = hit ? h2 - d2 : 0.0f;
                 sum = fmaf(t * t, t, sum);
-                const uint32_t b = j + (uint32_t)u - lo;
+                const uint32_t b = j - l + (uint32_t)u;
                 if (hit && b < (uint32_t)kMaskBits) m |= 1u << b;
             }
         }
@@ -84,51 +86,35 @@ __device__ __forceinline__ void force_pair(const SceneConst& sc, const float4& p
     f.vz = fmaf(vv, vj.z - vi.z, f.vz);
 }
 
-__device__ __forceinline__ void force_one(const StepArgs& a, const SceneConst& sc, uint32_t k) {
+__device__ __forceinline__ uint32_t force_one(const StepArgs& a, const SceneConst& sc, uint32_t k,
+                                              __amdgpu_buffer_rsrc_t rpos, __amdgpu_buffer_rsrc_t rvel,
+                                              __amdgpu_buffer_rsrc_t raux) {
     const float4 pi = a.pos_s[k];
     const float4 vi = a.vel_s[k];
     const float2 xi = a.aux[k];
-    const int cx = cell_coord(pi.x, sc.bmin[0], sc.inv_cs, sc.G[0]);
-    const int cy = cell_coord(pi.y, sc.bmin[1], sc.inv_cs, sc.G[1]);
-    const int cz = cell_coord(pi.z, sc.bmin[2], sc.inv_cs, sc.G[2]);
+    const RowGeom g = row_geom(sc, pi.x, pi.y, pi.z);
+    uint32_t lo[kRows], hi[kRows], mk[kRows];
+#pragma unroll
+    for (int row = 0; row < kRows; ++row) {
+        row_window(a.cstart, sc, g, row % 3 - 1, row / 3 - 1, lo[row], hi[row]);
+        mk[row] = a.mask[(size_t)row * a.n + k];
+    }
+    {
+        const uint32_t b = k - lo[4];  // own bit, in the centre row
+        if (b < (uint32_t)kMaskBits) mk[4] &= ~(1u << b);
+    }
     ForceAcc f = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int row = 0; row < kRows; ++row) {
-        const int dz = row / 3 - 1, dy = row % 3 - 1;
-        uint32_t lo, hi;
-        row_range(a.cstart, sc, cx, cy + dy, cz + dz, lo, hi);
-        uint32_t m = a.mask[(size_t)row * a.n + k];
-        if (row == 4) {
-            const uint32_t b = k - lo;
-            if (b < (uint32_t)kMaskBits) m &= ~(1u << b);
-        }
-        if (m) {
-            // Software-pipelined: the next neighbour's loads are issued
-            // before the current neighbour's arithmetic.
-            uint32_t j = lo + (uint32_t)__builtin_ctz(m);
+        uint32_t m = mk[row];
+        const uint32_t l = lo[row];
+        while (m) {
+            const uint32_t j = l + (uint32_t)__builtin_ctz(m);
             m &= m - 1u;
-            float4 pj = a.pos_s[j], vj = a.vel_s[j];
-            float2 xj = a.aux[j];
-            while (true) {
-                const bool more = m != 0u;
-                float4 pn = pj, vn = vj;
-                float2 xn = xj;
-                if (more) {
-                    j = lo + (uint32_t)__builtin_ctz(m);
-                    m &= m - 1u;
-                    pn = a.pos_s[j];
-                    vn = a.vel_s[j];
-                    xn = a.aux[j];
-                }
-                force_pair(sc, pi, vi, xi.x, pj, vj, xj, f);
-                if (!more) break;
-                pj = pn;
-                vj = vn;
-                xj = xn;
-            }
+            force_pair(sc, pi, vi, xi.x, load4(rpos, j << 4), load4(rvel, j << 4), load2(raux, j << 3), f);
         }
-        if (hi > lo + (uint32_t)kMaskBits) {
-            for (uint32_t j = lo + (uint32_t)kMaskBits; j < hi; ++j) {
+        if (hi[row] > l + (uint32_t)kMaskBits) {
+            for (uint32_t j = l + (uint32_t)kMaskBits; j < hi[row]; ++j) {
                 if (j == k) continue;
                 const float4 pj = a.pos_s[j];
                 if (dist2(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z) < sc.h2)
@@ -156,32 +142,42 @@ __device__ __forceinline__ void force_one(const StepArgs& a, const SceneConst& s
     a.pos_o[k] = make_float4(p[0], p[1], p[2], 0.0f);
     a.vel_o[k] = make_float4(v[0], v[1], v[2], 0.0f);
     a.id_o[k] = a.id_s[k];
-    const uint32_t key = cell_key(sc, p[0], p[1], p[2]);
-    a.key0[k] = key;
-    a.rank0[k] = atomicAdd(&a.count[key], 1u);
+    return cell_key(sc, p[0], p[1], p[2]);
 }
 
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void density_kernel(StepArgs a) {
+    const __amdgpu_buffer_rsrc_t rpos = buffer_of(a.pos_s, a.n);
     uint32_t t0, t1;
     tile_span(a.ntiles, t0, t1);
     for (uint32_t t = t0; t < t1; ++t) {
         const uint2 ti = a.tiles[t];
         const SceneConst& sc = a.scenes[ti.x];
         const uint32_t k = ti.y + threadIdx.x;
-        if (k < sc.p_end) density_one(a, sc, k);
+        if (k < sc.p_end) density_one(a, sc, k, rpos);
     }
 }
 
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void force_kernel(StepArgs a) {
+    const __amdgpu_buffer_rsrc_t rpos = buffer_of(a.pos_s, a.n);
+    const __amdgpu_buffer_rsrc_t rvel = buffer_of(a.vel_s, a.n);
+    const __amdgpu_buffer_rsrc_t raux = buffer_of(a.aux, a.n);
     uint32_t t0, t1;
     tile_span(a.ntiles, t0, t1);
     for (uint32_t t = t0; t < t1; ++t) {
         const uint2 ti = a.tiles[t];
         const SceneConst& sc = a.scenes[ti.x];
         const uint32_t k = ti.y + threadIdx.x;
-        if (k < sc.p_end) force_one(a, sc, k);
+        const bool active = k < sc.p_end;
+        uint32_t key = 0;
+        if (active) key = force_one(a, sc, k, rpos, rvel, raux);
+        // Next step's key and its slot in the per-cell count (whole wave).
+        const uint32_t slot = count_slot(a.count, key, active);
+        if (active) {
+            a.key0[k] = key;
+            a.rank0[k] = slot;
+        }
     }
 }
 
@@ -195,11 +191,16 @@ __global__ void hash_kernel(const float4* __restrict__ pos, uint32_t* __restrict
         const uint2 ti = tiles[t];
         const SceneConst& sc = scenes[ti.x];
         const uint32_t k = ti.y + threadIdx.x;
-        if (k < sc.p_end) {
+        const bool active = k < sc.p_end;
+        uint32_t key = 0;
+        if (active) {
             const float4 p = pos[k];
-            const uint32_t key = cell_key(sc, p.x, p.y, p.z);
+            key = cell_key(sc, p.x, p.y, p.z);
+        }
+        const uint32_t slot = count_slot(count, key, active);
+        if (active) {
             key0[k] = key;
-            rank0[k] = atomicAdd(&count[key], 1u);
+            rank0[k] = slot;
         }
     }
 }
@@ -216,15 +217,12 @@ __global__ void neighbors_kernel(StepArgs a, const uint32_t* __restrict__ offset
         const uint32_t k = ti.y + threadIdx.x;
         if (k >= sc.p_end) continue;
         const float4 pi = a.pos_s[k];
-        const int cx = cell_coord(pi.x, sc.bmin[0], sc.inv_cs, sc.G[0]);
-        const int cy = cell_coord(pi.y, sc.bmin[1], sc.inv_cs, sc.G[1]);
-        const int cz = cell_coord(pi.z, sc.bmin[2], sc.inv_cs, sc.G[2]);
+        const RowGeom g = row_geom(sc, pi.x, pi.y, pi.z);
         uint32_t c = 0;
         const uint32_t o = FILL ? offsets[k] : 0u;
         for (int row = 0; row < kRows; ++row) {
-            const int dz = row / 3 - 1, dy = row % 3 - 1;
             uint32_t lo, hi;
-            row_range(a.cstart, sc, cx, cy + dy, cz + dz, lo, hi);
+            row_window(a.cstart, sc, g, row % 3 - 1, row / 3 - 1, lo, hi);
             uint32_t m = a.mask[(size_t)row * a.n + k];
             while (m) {
                 const uint32_t b = (uint32_t)__builtin_ctz(m);

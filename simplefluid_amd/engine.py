@@ -53,7 +53,7 @@ EXPORTS = (
     "sph_get_tuning", "sph_set_tuning", "sph_debug_constants", "sph_debug_get_state",
     "sph_debug_set_state", "sph_debug_keep_acc", "sph_debug_step_outputs",
     "sph_debug_num_cells", "sph_debug_cells", "sph_debug_neighbors", "sph_set_sort_mode",
-    "sph_get_sort_mode",
+    "sph_get_sort_mode", "sph_set_cell_subdivision",
 )
 SORT_CELL_COUNT = 0
 SORT_RADIX = 1
@@ -143,6 +143,7 @@ def load_library(path: str = LIB_PATH):
         "sph_debug_neighbors": (i32, [H, vp, vp, C.c_uint64, P(C.c_uint64)]),
         "sph_set_sort_mode": (i32, [H, i32]),
         "sph_get_sort_mode": (i32, [H]),
+        "sph_set_cell_subdivision": (i32, [H, i32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -331,6 +332,9 @@ class Engine:
     def get_sort_mode(self) -> int:
         return int(self._L.sph_get_sort_mode(self._h))
 
+    def set_cell_subdivision(self, xsub: int):
+        self._chk(self._L.sph_set_cell_subdivision(self._h, int(xsub)))
+
     # -- rollout adjoint ------------------------------------------------------
     def rollout_forward(self, steps: int, checkpoint_every: int = 20):
         self._chk(self._L.sph_rollout_forward(self._h, int(steps), int(checkpoint_every)))
@@ -352,13 +356,14 @@ class Engine:
     # -- test-only views ------------------------------------------------------
     def debug_constants(self, scene: int = 0):
         out = np.zeros(14, np.float32)
-        G = np.zeros(3, np.int32)
+        G = np.zeros(4, np.int32)
         self._chk(self._L.sph_debug_constants(self._h, int(scene), _p(out), _p(G)))
         names = ["h", "h2", "inv_cs", "mass", "cD", "cP", "cV", "inv_rho0"]
         d = {k: out[i] for i, k in enumerate(names)}
         d["lo"] = out[8:11].copy()
         d["hi"] = out[11:14].copy()
-        d["G"] = G
+        d["G"] = G[:3].copy()
+        d["xsub"] = int(G[3])
         return d
 
     def debug_get_state(self):

@@ -23,9 +23,9 @@ def say(*a):
     print(*a, flush=True)
 
 
-def to_oracle(p):
+def to_oracle(p, xsub=1):
     import oracle
-    return oracle.from_params(p)
+    return oracle.from_params(p, xsub)
 
 
 @pytest.fixture(scope="module")
@@ -46,7 +46,7 @@ def step_and_compare(E, e, params, label, threads=16):
     """State of e -> oracle; one step on both; compare every intermediate."""
     import oracle
     xs, vs, ids = e.debug_get_state()
-    o = oracle.Oracle(to_oracle(params), threads=threads)
+    o = oracle.Oracle(to_oracle(params, e.debug_constants()["xsub"]), threads=threads)
     o.record_neighbors(True)
     o.set_state(xs.astype(np.float64), vs.astype(np.float64), ids)
     e.debug_keep_acc(True)
@@ -88,8 +88,10 @@ def test_constants_match_oracle(E):
     from simplefluid_amd import configs
     for cfg in (configs.dambreak_65k(), configs.double_dambreak_1m()):
         e = E.Engine(cfg["params"])
+        e.set_particles(cfg["pos"])
         c = e.debug_constants()
-        oc = oracle.Oracle(to_oracle(cfg["params"])).constants()
+        assert c["xsub"] == 4
+        oc = oracle.Oracle(to_oracle(cfg["params"], c["xsub"])).constants()
         for k in ("h", "h2", "inv_cs", "mass", "cD", "cP", "cV", "inv_rho0"):
             assert np.float32(c[k]) == np.float32(oc[k]), k
         assert np.array_equal(c["G"], oc["G"])
@@ -177,6 +179,18 @@ def test_sort_modes_agree(E):
         out.append((e.get_particles(), e.debug_step_outputs()["perm"]))
         e.close()
     assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
+@pytest.mark.parametrize("xsub", [1, 2, 4])
+def test_cell_subdivision_parity(E, xsub):
+    """Every x subdivision gives oracle-exact neighbour sets and keys."""
+    from simplefluid_amd import configs
+    cfg = configs.dambreak_65k(jitter=1e-6, seed=12)
+    e = make_engine(E, cfg["params"], cfg["pos"])
+    e.set_cell_subdivision(xsub)
+    assert e.debug_constants()["xsub"] == xsub
+    e.advance_steps(30)
+    step_and_compare(E, e, cfg["params"], f"xsub {xsub}")
 
 
 def test_get_particles_original_order(E):
@@ -284,7 +298,7 @@ def test_batched_parity_vs_oracle(E):
     xg, _, ig = b.debug_get_state()
     for i, s in enumerate(ens["scenes"]):
         sl = slice(off[i], off[i + 1])
-        o = oracle.Oracle(to_oracle(s["params"]), threads=8)
+        o = oracle.Oracle(to_oracle(s["params"], b.debug_constants(i)["xsub"]), threads=8)
         o.set_state(xs[sl].astype(np.float64), vs[sl].astype(np.float64), ids[sl] - off[i])
         o.step()
         xo, _, io = o.state()

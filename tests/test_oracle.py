@@ -88,6 +88,32 @@ def test_neighbor_sets_equal_bruteforce(oracle_lib, jitter):
     assert all(k in set(lst[off[k]:off[k + 1]]) for k in range(0, n, 97))
 
 
+@pytest.mark.parametrize("xsub", [2, 4])
+def test_subdivided_grid_neighbors_and_density(oracle_lib, xsub):
+    """x cells of cs/xsub: same neighbour sets as brute force, and density
+    equal to the xsub = 1 grid up to summation order."""
+    import oracle
+    pos = lattice((9, 11, 8), jitter=0.001, seed=6, origin=(0.21, 0.05, 0.4))
+    res = {}
+    for xs in (1, xsub):
+        p = oracle.make_params(x_subdivision=xs)
+        o = oracle.Oracle(p)
+        o.record_neighbors(True)
+        o.set_state(pos)
+        o.step()
+        d = o.step_debug()
+        off, lst = o.neighbors()
+        sp = pos[d["perm"]].astype(np.float32)
+        boff, blst = oracle.bruteforce_neighbors(sp, p.kernel_radius)
+        assert np.array_equal(off, boff)
+        for k in range(0, len(pos), 7):
+            assert np.array_equal(np.sort(lst[off[k]:off[k + 1]]), blst[boff[k]:boff[k + 1]])
+        assert o.constants()["G"][0] == 50 * xs
+        rho = np.empty(len(pos)); rho[d["perm"]] = d["rho"]
+        res[xs] = rho
+    assert np.allclose(res[1], res[xsub], rtol=1e-6)
+
+
 def test_lattice_interior_neighbour_count(oracle_lib):
     """Spacing h/2: 27 lattice points strictly inside radius h (33 with <=);
     float ties at exactly h land on either side, so interior counts lie in [27, 33]."""
