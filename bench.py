@@ -108,7 +108,7 @@ def load_traffic(kernel):
 def cpu_baseline(cfg, budget_s):
     import oracle
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
-    o = oracle.Oracle(oracle.from_params(cfg["params"]), threads=threads)
+    o = oracle.Oracle(oracle.from_params(cfg["params"], 4), threads=threads)
     o.set_state(cfg["pos"].astype(np.float64))
     o.step(1)  # warm-up
     steps, spent = 0, 0.0
