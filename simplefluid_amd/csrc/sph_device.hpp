@@ -128,15 +128,28 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_of(const T* p, uint32_t
     return __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, (int)(n * (uint32_t)sizeof(T)), 0x00020000);
 }
 
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+
+// Components go through named vector members: reading elements of the
+// builtin's result by subscript inside __builtin_bit_cast loses all but one
+// dword of the load with this compiler.
 __device__ __forceinline__ float4 load4(__amdgpu_buffer_rsrc_t r, uint32_t byte_off) {
-    const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)byte_off, 0, 0);
-    return make_float4(__builtin_bit_cast(float, v[0]), __builtin_bit_cast(float, v[1]),
-                       __builtin_bit_cast(float, v[2]), __builtin_bit_cast(float, v[3]));
+    const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)byte_off, 0, 0);
+    float4 f;
+    f.x = __uint_as_float(v.x);
+    f.y = __uint_as_float(v.y);
+    f.z = __uint_as_float(v.z);
+    f.w = __uint_as_float(v.w);
+    return f;
 }
 
 __device__ __forceinline__ float2 load2(__amdgpu_buffer_rsrc_t r, uint32_t byte_off) {
-    const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, (int)byte_off, 0, 0);
-    return make_float2(__builtin_bit_cast(float, v[0]), __builtin_bit_cast(float, v[1]));
+    const u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(r, (int)byte_off, 0, 0);
+    float2 f;
+    f.x = __uint_as_float(v.x);
+    f.y = __uint_as_float(v.y);
+    return f;
 }
 
 __device__ __forceinline__ uint32_t cell_key(const SceneConst& sc, float x, float y, float z) {
