@@ -23,9 +23,10 @@ def case(n3=(16, 16, 16), seed=0, B=50.0, mu=0.01, tscale=0.003):
     return p, pos, vel, target
 
 
-def gpu_grads(p, pos, vel, target, steps, every):
+def gpu_grads(p, pos, vel, target, steps, every, xsub=0):
     from simplefluid_amd import engine
     e = engine.Engine(p)
+    e.set_cell_subdivision(xsub)
     e.set_particles(pos, vel)
     e.make_ready()
     e.rollout_forward(steps, every)
@@ -101,6 +102,7 @@ def test_batched_adjoint_equals_single_scene():
     cases = [case(n3=(8, 10, 8), seed=s, B=40.0 + 20 * s, mu=0.01 * (s + 1)) for s in range(3)]
     steps, every = 16, 5
     b = engine.BatchEngine([c[0] for c in cases], [len(c[1]) for c in cases])
+    b.set_cell_subdivision(4)   # same grid as the single-scene handles below
     b.set_particles(np.concatenate([c[1] for c in cases]), np.concatenate([c[2] for c in cases]))
     b.make_ready()
     b.rollout_forward(steps, every)
@@ -113,7 +115,7 @@ def test_batched_adjoint_equals_single_scene():
         g[sl] = 2.0 / (3 * (off[i + 1] - off[i])) * (xT[sl] - tgt[sl])
     dB, dmu, dv0 = b.rollout_backward(g.astype(np.float32))
     for i, (p, pos, vel, target) in enumerate(cases):
-        _, sB, smu, sdv0, _, _ = gpu_grads(p, pos, vel, target, steps, every)
+        _, sB, smu, sdv0, _, _ = gpu_grads(p, pos, vel, target, steps, every, xsub=4)
         assert dB[i] == np.float32(sB) and dmu[i] == np.float32(smu)
         assert np.array_equal(dv0[off[i]:off[i + 1]], sdv0.astype(np.float32))
 

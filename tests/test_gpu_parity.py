@@ -269,13 +269,17 @@ def test_batched_equals_per_scene(E):
     params = [s["params"] for s in ens["scenes"]]
     counts = [s["pos"].shape[0] for s in ens["scenes"]]
     b = E.BatchEngine(params, counts)
+    b.set_cell_subdivision(2)   # one grid for the batch and the single scenes
     b.set_particles(np.concatenate([s["pos"] for s in ens["scenes"]]))
     b.make_ready()
     b.advance_steps(40)
     xb = b.get_particles()
     off = b.scene_offsets()
     for i, s in enumerate(ens["scenes"]):
-        e = make_engine(E, s["params"], s["pos"])
+        e = E.Engine(s["params"])
+        e.set_cell_subdivision(2)
+        e.set_particles(s["pos"])
+        e.make_ready()
         e.advance_steps(40)
         assert np.array_equal(e.get_particles(), xb[off[i]:off[i + 1]]), f"scene {i}"
         e.close()
