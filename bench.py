@@ -214,6 +214,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--only", choices=["plain", "batched", "adjoint"], default=None,
+                    help="run one leg alone (profiling)")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tuning", type=str, default="",
                     help="density,force,reorder blocks and sort items, e.g. 512,256,64,16 (skips autotune)")
@@ -222,6 +224,12 @@ def main():
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy()
+        return
+    if args.only == "batched":
+        print(json.dumps({"batched": batched(args)}), flush=True)
+        return
+    if args.only == "adjoint":
+        print(json.dumps({"adjoint": adjoint(args)}), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported by this build "

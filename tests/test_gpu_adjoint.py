@@ -112,7 +112,7 @@ def test_batched_adjoint_equals_single_scene():
     g = np.zeros_like(xT)
     for i in range(3):
         sl = slice(off[i], off[i + 1])
-        g[sl] = 2.0 / (3 * (off[i + 1] - off[i])) * (xT[sl] - tgt[sl])
+        g[sl] = 2 * (1.0 / (3 * (off[i + 1] - off[i]))) * (xT[sl] - tgt[sl])   # as gpu_grads forms it
     dB, dmu, dv0 = b.rollout_backward(g.astype(np.float32))
     for i, (p, pos, vel, target) in enumerate(cases):
         _, sB, smu, sdv0, _, _ = gpu_grads(p, pos, vel, target, steps, every, xsub=4)
