@@ -36,7 +36,7 @@ constexpr uint32_t kFlagInc = 2u << 30;
 constexpr uint32_t kValMask = (1u << 30) - 1u;
 constexpr uint32_t kSpinLimit = 1u << 22;
 constexpr int kHistItems = 16;
-constexpr int kScanItems = 16;
+constexpr int kScanItems = 32;
 constexpr int kScanTile = kSortBlock * kScanItems;
 
 __device__ __forceinline__ uint32_t ld_agent(const uint32_t* p) {
@@ -94,6 +94,39 @@ __device__ __forceinline__ uint32_t look_back(const uint32_t* status, uint32_t s
     return prefix;
 }
 
+// Wave-wide look-back for one value per tile: lane l probes tile
+// (tile - 1 - l) of the current 64-tile window; the closest inclusive prefix
+// ends the walk, otherwise all 64 aggregates are added and the window moves
+// back.  Tiles before 0 read as an inclusive 0.  Called by all 64 lanes of
+// one wave; returns the exclusive prefix of `tile`.
+__device__ __forceinline__ uint32_t wave_look_back(const uint32_t* status, uint32_t tile, uint32_t* err) {
+    const int lane = threadIdx.x & 63;
+    uint32_t prefix = 0;
+    int base = (int)tile - 1;
+    uint32_t spins = 0;
+    while (true) {
+        const int t = base - lane;
+        const uint32_t s = t >= 0 ? ld_agent(status + t) : kFlagInc;
+        if (__any((s & ~kValMask) == 0u)) {
+            if (++spins > kSpinLimit) {
+                if (lane == 0) atomicOr(err, 1u);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+            continue;
+        }
+        const uint64_t inc = __ballot((s & ~kValMask) == kFlagInc);
+        const int first = inc ? __ffsll((long long)inc) - 1 : 64;
+        uint32_t v = lane <= first ? (s & kValMask) : 0u;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+        prefix += v;
+        if (inc) break;
+        base -= 64;
+    }
+    return prefix;
+}
+
 // ---------------------------------------------------------------- cell count
 
 __global__ __launch_bounds__(kSortBlock) void cell_scan_kernel(uint32_t* __restrict__ count,
@@ -130,16 +163,16 @@ __global__ __launch_bounds__(kSortBlock) void cell_scan_kernel(uint32_t* __restr
     for (int i = 0; i < kScanItems; ++i) sum += v[i];
     uint32_t total = 0;
     const uint32_t excl = block_exclusive_scan(sum, s_w, &total);
-    if (tid == 0) {
+    if (tid < 64) {
         uint32_t prefix = 0;
         if (tile == 0) {
-            st_agent(status, kFlagInc | total);
+            if (tid == 0) st_agent(status, kFlagInc | total);
         } else {
-            st_agent(status + tile, kFlagAgg | total);
-            prefix = look_back(status, 1u, tile, err);
-            st_agent(status + tile, kFlagInc | (prefix + total));
+            if (tid == 0) st_agent(status + tile, kFlagAgg | total);
+            prefix = wave_look_back(status, tile, err);
+            if (tid == 0) st_agent(status + tile, kFlagInc | (prefix + total));
         }
-        s_prefix = prefix;
+        if (tid == 0) s_prefix = prefix;
     }
     __syncthreads();
     uint32_t run = s_prefix + excl;
