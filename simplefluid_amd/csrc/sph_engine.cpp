@@ -745,11 +745,14 @@ int sph_set_cell_subdivision(sph_handle* h, int xsub) {
     if (!h) return SPH_E_INVALID;
     if (xsub != 0 && xsub != 1 && xsub != 2 && xsub != 4) return fail(h, SPH_E_INVALID, "x subdivision must be 0, 1, 2 or 4");
     h->xsub_override = xsub;
-    if (!h->has_particles) return SPH_OK;
+    if (!h->has_particles && !h->batched) return SPH_OK;  // applied when particles are set
     // Re-grid keeping the state (it stays in its current order; keys are rebuilt).
-    const int rc = configure(h, h->params, h->scene_n);
+    int rc = configure(h, h->params, h->scene_n);
     if (rc != SPH_OK) return rc;
     h->keys_valid = false;
+    rc = ensure_buffers(h);
+    if (rc != SPH_OK) return rc;
+    if (!h->has_particles) return SPH_OK;
     h->ready = false;
     return sph_make_ready(h);
 }

@@ -86,18 +86,9 @@ __device__ __forceinline__ void force_pair(const SceneConst& sc, const float4& p
     f.vz = fmaf(vv, vj.z - vi.z, f.vz);
 }
 
-constexpr int kListCap = 64;   // flattened neighbour entries per lane in LDS
-
-// Force sum for particle k.  The nine row masks are first decoded into a
-// per-lane LDS list of (row, bit) entries, then one flat loop visits them in
-// the canonical order (rows in order, bits ascending): lanes of a wave then
-// loop max(total hits) times instead of sum over rows of max(row hits).
-// A lane with more than kListCap hits walks its rows directly instead.
-template <int BLOCK>
 __device__ __forceinline__ uint32_t force_one(const StepArgs& a, const SceneConst& sc, uint32_t k,
                                               __amdgpu_buffer_rsrc_t rpos, __amdgpu_buffer_rsrc_t rvel,
-                                              __amdgpu_buffer_rsrc_t raux, uint16_t* s_list, uint32_t* s_lo) {
-    const uint32_t tid = threadIdx.x;
+                                              __amdgpu_buffer_rsrc_t raux) {
     const float4 pi = a.pos_s[k];
     const float4 vi = a.vel_s[k];
     const float2 xi = a.aux[k];
@@ -112,45 +103,18 @@ __device__ __forceinline__ uint32_t force_one(const StepArgs& a, const SceneCons
         const uint32_t b = k - lo[4];  // own bit, in the centre row
         if (b < (uint32_t)kMaskBits) mk[4] &= ~(1u << b);
     }
-    uint32_t total = 0;
+    ForceAcc f = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int row = 0; row < kRows; ++row) {
-        total += (uint32_t)__builtin_popcount(mk[row]);
-        s_lo[row * BLOCK + tid] = lo[row];
-    }
-    ForceAcc f = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    if (total <= (uint32_t)kListCap) {
-        uint32_t cnt = 0;
-#pragma unroll
-        for (int row = 0; row < kRows; ++row) {
-            uint32_t m = mk[row];
-            while (m) {
-                s_list[cnt * BLOCK + tid] = (uint16_t)((row << 5) | __builtin_ctz(m));
-                m &= m - 1u;
-                ++cnt;
-            }
-        }
-        for (uint32_t e = 0; e < cnt; ++e) {
-            const uint32_t ent = s_list[e * BLOCK + tid];
-            const uint32_t j = s_lo[(ent >> 5) * BLOCK + tid] + (ent & 31u);
+        uint32_t m = mk[row];
+        const uint32_t l = lo[row];
+        while (m) {
+            const uint32_t j = l + (uint32_t)__builtin_ctz(m);
+            m &= m - 1u;
             force_pair(sc, pi, vi, xi.x, load4(rpos, j << 4), load4(rvel, j << 4), load2(raux, j << 3), f);
         }
-    } else {
-#pragma unroll
-        for (int row = 0; row < kRows; ++row) {
-            uint32_t m = mk[row];
-            while (m) {
-                const uint32_t j = lo[row] + (uint32_t)__builtin_ctz(m);
-                m &= m - 1u;
-                force_pair(sc, pi, vi, xi.x, load4(rpos, j << 4), load4(rvel, j << 4), load2(raux, j << 3), f);
-            }
-        }
-    }
-    // Rows longer than the mask: the tail is tested again here.
-#pragma unroll
-    for (int row = 0; row < kRows; ++row) {
-        if (hi[row] > lo[row] + (uint32_t)kMaskBits) {
-            for (uint32_t j = lo[row] + (uint32_t)kMaskBits; j < hi[row]; ++j) {
+        if (hi[row] > l + (uint32_t)kMaskBits) {
+            for (uint32_t j = l + (uint32_t)kMaskBits; j < hi[row]; ++j) {
                 if (j == k) continue;
                 const float4 pj = a.pos_s[j];
                 if (dist2(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z) < sc.h2)
@@ -196,8 +160,6 @@ __global__ __launch_bounds__(BLOCK) void density_kernel(StepArgs a) {
 
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void force_kernel(StepArgs a) {
-    __shared__ uint16_t s_list[kListCap * BLOCK];
-    __shared__ uint32_t s_lo[kRows * BLOCK];
     const __amdgpu_buffer_rsrc_t rpos = buffer_of(a.pos_s, a.n);
     const __amdgpu_buffer_rsrc_t rvel = buffer_of(a.vel_s, a.n);
     const __amdgpu_buffer_rsrc_t raux = buffer_of(a.aux, a.n);
@@ -209,7 +171,7 @@ __global__ __launch_bounds__(BLOCK) void force_kernel(StepArgs a) {
         const uint32_t k = ti.y + threadIdx.x;
         const bool active = k < sc.p_end;
         uint32_t key = 0;
-        if (active) key = force_one<BLOCK>(a, sc, k, rpos, rvel, raux, s_list, s_lo);
+        if (active) key = force_one(a, sc, k, rpos, rvel, raux);
         // Next step's key and its slot in the per-cell count (whole wave).
         const uint32_t slot = count_slot(a.count, key, active);
         if (active) {
