@@ -24,7 +24,9 @@
  *   - Device memory belongs to the handle.  A handle is used by one thread at
  *     a time (the reference's single worker thread, Source/Simulator.cpp:29).
  *   - Every int-returning call returns SPH_OK (0) or a negative SPH_E_* code;
- *     sph_last_error() gives the message.  No exception crosses the ABI.
+ *     sph_last_error(h) gives the message (sph_last_error(NULL): the last
+ *     failure on this thread without a handle, e.g. in sph_create).  No
+ *     exception crosses the ABI.
  *   - Only sph_get_*, sph_synchronize and the debug fetches wait for the GPU.
  */
 #ifndef SPH_ENGINE_H

@@ -17,8 +17,13 @@ sph_handle::~sph_handle() {
 
 namespace sph {
 
+// Message of the last failure on this thread that had no handle to carry it
+// (sph_create, sph_batch_create); sph_last_error(NULL) returns it.
+thread_local std::string t_last_error = "no error";
+
 int fail(sph_handle* h, int code, const std::string& msg) {
     if (h) h->last_error = msg;
+    else t_last_error = msg;
     return code;
 }
 
@@ -368,7 +373,7 @@ static int new_handle(sph_handle** out) {
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete h;
-        return SPH_E_HIP;
+        return fail(nullptr, SPH_E_HIP, std::string("no usable HIP device: ") + hipGetErrorString(e));
     }
     *out = h;
     return SPH_OK;
@@ -410,7 +415,7 @@ int sph_create(const sph_params* p, sph_handle** out) {
     rc = derive_scene(*p, tmp, err, 1);
     if (rc != SPH_OK) {
         delete h;
-        return rc;
+        return fail(nullptr, rc, err);
     }
     h->params.assign(1, *p);
     h->scene_n.assign(1, 0);
@@ -606,7 +611,7 @@ int sph_synchronize(sph_handle* h) {
     return sync(h);
 }
 
-const char* sph_last_error(const sph_handle* h) { return h ? h->last_error.c_str() : "null handle"; }
+const char* sph_last_error(const sph_handle* h) { return h ? h->last_error.c_str() : t_last_error.c_str(); }
 
 int sph_set_use_graph(sph_handle* h, int on) {
     if (!h) return SPH_E_INVALID;
@@ -629,14 +634,11 @@ int sph_batch_create(const sph_scene_desc* scenes, uint32_t nscenes, sph_handle*
     h->batched = true;
     h->persistent = true;
     rc = configure(h, ps, ns);
+    if (rc == SPH_OK) rc = ensure_buffers(h);
     if (rc != SPH_OK) {
+        const std::string msg = h->last_error;
         delete h;
-        return rc;
-    }
-    rc = ensure_buffers(h);
-    if (rc != SPH_OK) {
-        delete h;
-        return rc;
+        return fail(nullptr, rc, msg);
     }
     *out = h;
     return SPH_OK;

@@ -227,7 +227,7 @@ class Engine:
             h = C.c_void_p()
             rc = self._L.sph_create(C.byref(params if params is not None else make_params()), C.byref(h))
             if rc != SPH_OK:
-                raise SPHError(rc, "sph_create failed (invalid parameters?)")
+                raise SPHError(rc, "sph_create: " + self._L.sph_last_error(None).decode())
             self._h = h
 
     def close(self):
@@ -426,7 +426,7 @@ class BatchEngine(Engine):
         h = C.c_void_p()
         rc = L.sph_batch_create(descs, len(params_list), C.byref(h))
         if rc != SPH_OK:
-            raise SPHError(rc, "sph_batch_create failed")
+            raise SPHError(rc, "sph_batch_create: " + L.sph_last_error(None).decode())
         super().__init__(_handle=h)
 
     def scene_offsets(self) -> np.ndarray:

@@ -114,6 +114,40 @@ def test_dambreak_65k_parity_along_rollout(E):
         step_and_compare(E, e, cfg["params"], f"dambreak_65k after {k}")
 
 
+def test_dambreak_65k_drift_vs_fp64(E):
+    """The full 65k dam-break rollout (1000 steps of 1e-4 s): drift of the
+    GPU FP32 trajectory from the FP64 oracle, stated beside the FP32 oracle's
+    own drift from FP64 over the same rollout (distances in units of h)."""
+    import oracle
+    from simplefluid_amd import configs
+    cfg = configs.dambreak_65k()
+    steps = cfg["steps"]
+    e = make_engine(E, cfg["params"], cfg["pos"])
+    xsub = e.debug_constants()["xsub"]
+    e.advance_steps(steps)
+    xg = e.get_particles().astype(np.float64)
+    e.close()
+    ref = {}
+    for fp64 in (True, False):
+        say(f"[drift] oracle fp64={fp64} {steps} steps")
+        o = oracle.Oracle(to_oracle(cfg["params"], xsub), fp64=fp64, threads=16)
+        o.set_state(cfg["pos"].astype(np.float64))
+        o.step(steps)
+        ref[fp64] = o.state_by_id()[0]
+    h = float(cfg["params"].kernel_radius)
+
+    def drift(a, b):
+        d = np.linalg.norm(a - b, axis=1) / h
+        return d.max(), float(np.sqrt((d * d).mean()))
+
+    gmax, grms = drift(xg, ref[True])
+    omax, orms = drift(ref[False], ref[True])
+    say(f"[drift] after {steps} steps vs FP64 oracle (units of h): GPU max {gmax:.3e} rms {grms:.3e}; "
+        f"FP32 oracle max {omax:.3e} rms {orms:.3e}")
+    assert np.isfinite(xg).all()
+    assert grms <= max(3.0 * orms, 0.05), "GPU drifts from FP64 well beyond FP32 round-off growth"
+
+
 def test_double_dambreak_subset_parity(E):
     """The 1M configuration's physics on a 1/8 slab (131,072 particles) with
     its jitter and box, checked after 40 steps."""

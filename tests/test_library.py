@@ -106,3 +106,13 @@ def test_synthetic_configs_shapes():
     assert min(ns) >= 3000 and max(ns) <= 70000
     for s in e["scenes"]:
         assert (s["pos"] < 1.0).all()
+
+
+@pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a GPU driver is present")
+def test_no_gpu_fails_loudly(lib):
+    """Without a GPU the engine refuses to run (no CPU fallback) and says why."""
+    from simplefluid_amd import engine
+    with pytest.raises(engine.SPHError) as ei:
+        engine.Engine(engine.make_params())
+    assert ei.value.code == engine.SPH_E_HIP
+    assert "HIP device" in str(ei.value)
