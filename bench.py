@@ -218,7 +218,8 @@ def main():
                     help="run one leg alone (profiling)")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tuning", type=str, default="",
-                    help="density,force,reorder blocks and sort items, e.g. 512,256,64,16 (skips autotune)")
+                    help="density,force,reorder blocks, sort items [,persistent 0/1], e.g. 256,128,512,4,0 "
+                         "(skips autotune)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run a 256 MiB device copy (known bytes) for counter calibration, then exit")
     args = ap.parse_args()

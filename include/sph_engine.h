@@ -158,10 +158,11 @@ int  sph_profile_read(sph_handle* h, double* ms_per_pass, uint32_t* steps);
  * fastest (state is saved and restored).  Results are cached per process
  * keyed by (device arch, particle-count bucket, scene count).               */
 int  sph_autotune(sph_handle* h);
-/* Current choice: block sizes of density, force, reorder and the sort items
- * per thread (4 ints).                                                      */
-int  sph_get_tuning(const sph_handle* h, int* cfg4);
-int  sph_set_tuning(sph_handle* h, const int* cfg4);
+/* Current choice (5 ints): block sizes of density, force and reorder, the
+ * sort items per thread, and 1 when density and force run as persistent
+ * blocks taking tiles from per-XCD queues (0: one block per tile).          */
+int  sph_get_tuning(const sph_handle* h, int* cfg5);
+int  sph_set_tuning(sph_handle* h, const int* cfg5);
 /* Sort used to order particles by cell (no reference counterpart; both give
  * the same stable permutation and cell-start table):
  *   SPH_SORT_CELL_COUNT  one-digit (counting) radix sort over the whole cell
@@ -172,7 +173,8 @@ int  sph_set_tuning(sph_handle* h, const int* cfg4);
 int  sph_set_sort_mode(sph_handle* h, int mode);
 int  sph_get_sort_mode(const sph_handle* h);
 /* Cells are cs = h(1 + 2^-10) in y and z and cs/xsub in x (no reference
- * counterpart).  xsub = 0 picks automatically (4, 2 or 1 by cell budget);
+ * counterpart).  xsub = 0 picks automatically (the largest of 4, 2, 1 with
+ * at most 12 cells per particle, or 2^20 cells);
  * 1, 2, 4 force it.  Finer x cells let each neighbour row skip the x cells
  * outside the support sphere's chord.                                       */
 int  sph_set_cell_subdivision(sph_handle* h, int xsub);

@@ -35,27 +35,24 @@ __device__ __forceinline__ void tile_span(uint32_t ntiles, uint32_t& t0, uint32_
 template <class F>
 __device__ __forceinline__ void for_each_neighbor(const AdjArgs& a, const SceneConst& sc, uint32_t k,
                                                   const float4& pi, F&& f) {
+    const __amdgpu_buffer_rsrc_t rcs = buffer_of(a.cstart, a.ncells + 1u);
     const RowGeom g = row_geom(sc, pi.x, pi.y, pi.z);
 #pragma unroll
     for (int row = 0; row < kRows; ++row) {
         uint32_t lo, hi;
-        row_window(a.cstart, sc, g, row % 3 - 1, row / 3 - 1, lo, hi);
+        row_window(rcs, sc, g, row % 3 - 1, row / 3 - 1, lo, hi);
         uint32_t m = a.mask[(size_t)row * a.n + k];
         if (row == 4) {
-            const uint32_t b = k - lo;
+            const uint32_t b = mask_bit(lo, hi, k);
             if (b < (uint32_t)kMaskBits) m &= ~(1u << b);
         }
-        while (m) {
-            const uint32_t b = (uint32_t)__builtin_ctz(m);
-            m &= m - 1u;
-            f(lo + b);
-        }
-        if (hi > lo + (uint32_t)kMaskBits)
-            for (uint32_t j = lo + (uint32_t)kMaskBits; j < hi; ++j) {
-                if (j == k) continue;
+        row_neighbors(
+            lo, hi, m,
+            [&](uint32_t j) {
                 const float4 pj = a.pos_s[j];
-                if (dist2(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z) < sc.h2) f(j);
-            }
+                return j != k && dist2(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z) < sc.h2;
+            },
+            f);
     }
 }
 
@@ -104,12 +101,10 @@ __global__ void adj_force_kernel(AdjArgs a) {
         const uint32_t k = ti.y + threadIdx.x;
         if (k >= sc.p_end) continue;
         const float4 pi = a.pos_s[k], vi = a.vel_s[k], abi = a.abar[k];
-        const float2 xi = a.aux[k];
-        const float Pi = xi.x, qi = xi.y;
+        const float Pi = pi.w, qi = vi.w;  // p/rho^2 and 1/rho, stored by the density pass
         float gx = 0, gy = 0, gz = 0, hx = 0, hy = 0, hz = 0, Pb = 0, qb = 0, Ax = 0, Ay = 0, Az = 0;
         for_each_neighbor(a, sc, k, pi, [&](uint32_t j) {
             const float4 pj = a.pos_s[j], vj = a.vel_s[j], abj = a.abar[j];
-            const float2 xj = a.aux[j];
             const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
             const float d2 = dist2(rx, ry, rz);
             const float inv_r = d2 > 0.0f ? __builtin_amdgcn_rsqf(d2) : 0.0f;
@@ -120,10 +115,10 @@ __global__ void adj_force_kernel(AdjArgs a) {
             const float dphi_r = -tt * inv_r * inv_r * inv_r;
             const float dax = abi.x - abj.x, day = abi.y - abj.y, daz = abi.z - abj.z;
             const float dvx = vj.x - vi.x, dvy = vj.y - vi.y, dvz = vj.z - vi.z;
-            const float PP = sc.cP * (Pi + xj.x);
+            const float PP = sc.cP * (Pi + pj.w);
             const float dar = dax * rx + day * ry + daz * rz;
             const float dav = dax * dvx + day * dvy + daz * dvz;
-            const float cq = sc.cV * qi * xj.y;
+            const float cq = sc.cV * qi * vj.w;
             const float coefr = PP * dphi_r * dar - cq * inv_r * dav;
             gx += PP * phi * dax + coefr * rx;
             gy += PP * phi * day + coefr * ry;
@@ -132,10 +127,10 @@ __global__ void adj_force_kernel(AdjArgs a) {
             hy -= cq * hr * day;
             hz -= cq * hr * daz;
             Pb += sc.cP * phi * dar;
-            qb += sc.cV * xj.y * hr * dav;
-            Ax += hr * xj.y * dvx;
-            Ay += hr * xj.y * dvy;
-            Az += hr * xj.y * dvz;
+            qb += sc.cV * vj.w * hr * dav;
+            Ax += hr * vj.w * dvx;
+            Ay += hr * vj.w * dvy;
+            Az += hr * vj.w * dvz;
         });
         // EOS derivatives (clamped Tait, exponent 7).
         const float rho = a.rho[k];

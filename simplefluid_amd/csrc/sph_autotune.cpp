@@ -1,5 +1,6 @@
-// Autotuner: picks the block size of the density, force and reorder passes
-// and the keys per thread of the sort's digit passes by timing each candidate
+// Autotuner: picks the block size of the density, force and reorder passes,
+// the keys per thread of the sort's digit passes and whether density and
+// force run as persistent blocks over tile queues, by timing each candidate
 // with HIP events on the handle's own state.  The state is saved first and
 // restored before every trial, so all candidates run the same steps.
 // Results are cached per process, keyed by device arch, particle-count
@@ -23,7 +24,7 @@ std::string cache_key(const sph_handle* h) {
     int bucket = 0;
     while ((1u << bucket) < h->n && bucket < 31) ++bucket;
     return h->arch + ":" + std::to_string(bucket) + ":" + std::to_string(h->sc_h.size()) + ":" +
-           (h->persistent ? "p" : "g");
+           std::to_string(h->xsub);
 }
 
 struct Saved {
@@ -108,6 +109,20 @@ extern "C" int sph_autotune(sph_handle* h) {
         }
         *ax.field = best_v;
         if (rc != SPH_OK) break;
+    }
+    // Launch form of density and force, judged on their summed time.
+    if (rc == SPH_OK) {
+        double best_ms = 1e30;
+        int best_v = 0;
+        for (int c = 0; c < 2; ++c) {
+            best.persistent = c;
+            h->tune = best;
+            rc = trial(ms);
+            if (rc != SPH_OK) break;
+            const double t = ms[SPH_PASS_DENSITY] + ms[SPH_PASS_FORCE];
+            if (t < best_ms) { best_ms = t; best_v = c; }
+        }
+        best.persistent = best_v;
     }
     h->tune = best;
     h->prof = prof_was;

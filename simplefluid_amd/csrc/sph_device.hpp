@@ -5,12 +5,13 @@
 //   id[2]           uint32 x N   original particle id, state / sorted order
 //   key0            uint32 x N   cell key of the state positions (written by integrate)
 //   sort ping-pong  2 x (uint32 key + uint32 value) x N
-//   count           uint32 x C   particles per cell (atomic, built by the integrate pass)
+//   count           uint32 x C+T particles per cell, then per scan tile of 4096 cells
+//                                (atomic, built by the integrate pass)
 //   cstart          uint32 x C+1 lower-bound cell-start table: cstart[c] = #particles with key < c
 //   rho             float  x N   density (sorted order)
 //   aux             float2 x N   (p/rho^2, 1/rho) read by the force pass
-//   mask            uint32 x 9N  neighbour bits, row-major [row][particle]: bit b of row r
-//                                marks candidate lo_r + b as a neighbour (b < 32)
+//   mask            uint32 x 9N  neighbour bits, row-major [row][particle] (layout at
+//                                mask_span below)
 // All kernels are written for wave64 and 256 CUs in 8 XCDs.
 #pragma once
 
@@ -73,6 +74,32 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t bid, uint32_t nblocks) {
     return x * q + (x < r ? x : r) + i;
 }
 
+// Work over tiles.  A plain launch has one block per tile, in XCD-aware
+// order.  A persistent launch (queue != null, gridDim >= 8) gives each XCD an
+// equal contiguous share of the tiles; the XCD's blocks (blockIdx % 8) take
+// tiles from that share through an atomic counter, so costly tiles do not
+// hold up a fixed split.  The eight counters are zeroed before the launch.
+template <class Body>
+__device__ __forceinline__ void for_each_tile(uint32_t ntiles, uint32_t* __restrict__ queue, Body&& body) {
+    if (!queue) {
+        const uint32_t t = xcd_block(blockIdx.x, gridDim.x);
+        if (t < ntiles) body(t);
+        return;
+    }
+    __shared__ uint32_t s_next;
+    const uint32_t x = blockIdx.x & 7u;
+    const uint32_t t_beg = (uint32_t)(((uint64_t)ntiles * x) >> 3);
+    const uint32_t t_end = (uint32_t)(((uint64_t)ntiles * (x + 1u)) >> 3);
+    for (;;) {
+        if (threadIdx.x == 0) s_next = t_beg + atomicAdd(queue + x, 1u);
+        __syncthreads();
+        const uint32_t t = s_next;
+        __syncthreads();
+        if (t >= t_end) break;
+        body(t);
+    }
+}
+
 // Neighbour rows.  Cells are cs = h(1 + 2^-10) in y and z and cs/xsub in x.
 // The candidates of a particle lie in the 3x3 rows (dy, dz) of y/z cells
 // around its own; within a row, only x cells inside the chord of the support
@@ -83,9 +110,10 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t bid, uint32_t nblocks) {
 // of the neighbours (distances are under-estimated by 2^-10 cells and the
 // chord is widened by wmargin); the predicate d^2 < h^2 decides membership.
 struct RowGeom {
-    float fx;          // x coordinate in x cells
-    int cy, cz;
-    float qy[3], qz[3];  // squared y / z distance to rows dy, dz = -1, 0, +1 (cell units)
+    float fx;            // x coordinate in x cells
+    uint32_t row0;       // key of cell (x = 0, cy, cz)
+    float qy[3], qz[3];  // squared y / z distance to rows -1, 0, +1 (cell units);
+                         // 4 for a row outside the grid, which empties its window
 };
 
 __device__ __forceinline__ RowGeom row_geom(const SceneConst& sc, float x, float y, float z) {
@@ -93,33 +121,21 @@ __device__ __forceinline__ RowGeom row_geom(const SceneConst& sc, float x, float
     g.fx = (x - sc.bmin[0]) * sc.inv_csx;
     const float fy = (y - sc.bmin[1]) * sc.inv_cs;
     const float fz = (z - sc.bmin[2]) * sc.inv_cs;
-    g.cy = (int)fminf(fmaxf(fy, 0.0f), (float)(sc.G[1] - 1));
-    g.cz = (int)fminf(fmaxf(fz, 0.0f), (float)(sc.G[2] - 1));
+    const int cy = (int)fminf(fmaxf(fy, 0.0f), (float)(sc.G[1] - 1));
+    const int cz = (int)fminf(fmaxf(fz, 0.0f), (float)(sc.G[2] - 1));
     const float m = 0.0009765625f;
-    const float ry = fminf(fmaxf(fy - (float)g.cy, 0.0f), 1.0f);
-    const float rz = fminf(fmaxf(fz - (float)g.cz, 0.0f), 1.0f);
+    const float ry = fminf(fmaxf(fy - (float)cy, 0.0f), 1.0f);
+    const float rz = fminf(fmaxf(fz - (float)cz, 0.0f), 1.0f);
     const float ly = fmaxf(ry - m, 0.0f), uy = fmaxf(1.0f - ry - m, 0.0f);
     const float lz = fmaxf(rz - m, 0.0f), uz = fmaxf(1.0f - rz - m, 0.0f);
-    g.qy[0] = ly * ly; g.qy[1] = 0.0f; g.qy[2] = uy * uy;
-    g.qz[0] = lz * lz; g.qz[1] = 0.0f; g.qz[2] = uz * uz;
+    g.qy[0] = cy > 0 ? ly * ly : 4.0f;
+    g.qy[1] = 0.0f;
+    g.qy[2] = cy < sc.G[1] - 1 ? uy * uy : 4.0f;
+    g.qz[0] = cz > 0 ? lz * lz : 4.0f;
+    g.qz[1] = 0.0f;
+    g.qz[2] = cz < sc.G[2] - 1 ? uz * uz : 4.0f;
+    g.row0 = sc.cell_offset + ((uint32_t)cz * (uint32_t)sc.G[1] + (uint32_t)cy) * (uint32_t)sc.G[0];
     return g;
-}
-
-__device__ __forceinline__ void row_window(const uint32_t* __restrict__ cstart, const SceneConst& sc,
-                                           const RowGeom& g, int dy, int dz, uint32_t& lo, uint32_t& hi) {
-    lo = 0u;
-    hi = 0u;
-    const int y = g.cy + dy, z = g.cz + dz;
-    if (y < 0 || y >= sc.G[1] || z < 0 || z >= sc.G[2]) return;
-    const float w2 = sc.hc2 - (g.qy[dy + 1] + g.qz[dz + 1]);
-    if (!(w2 > 0.0f)) return;
-    const float wx = __builtin_amdgcn_sqrtf(w2) * sc.xsubf + sc.wmargin;
-    const float gmax = (float)(sc.G[0] - 1);
-    const int x0 = (int)fminf(fmaxf(g.fx - wx, 0.0f), gmax);
-    const int x1 = (int)fminf(fmaxf(g.fx + wx, 0.0f), gmax);
-    const uint32_t base = sc.cell_offset + ((uint32_t)z * (uint32_t)sc.G[1] + (uint32_t)y) * (uint32_t)sc.G[0];
-    lo = cstart[base + (uint32_t)x0];
-    hi = cstart[base + (uint32_t)x1 + 1u];
 }
 
 // Raw buffer resource over n elements of T: loads past the end return 0.
@@ -127,6 +143,51 @@ template <class T>
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_of(const T* p, uint32_t n) {
     return __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, (int)(n * (uint32_t)sizeof(T)), 0x00020000);
 }
+
+// Window [lo, hi) of row (dy, dz); rcs is a buffer over cstart.
+__device__ __forceinline__ void row_window(__amdgpu_buffer_rsrc_t rcs, const SceneConst& sc, const RowGeom& g,
+                                           int dy, int dz, uint32_t& lo, uint32_t& hi) {
+    lo = 0u;
+    hi = 0u;
+    const float w2 = sc.hc2 - (g.qy[dy + 1] + g.qz[dz + 1]);
+    if (!(w2 > 0.0f)) return;
+    const float wx = __builtin_amdgcn_sqrtf(w2) * sc.xsubf + sc.wmargin;
+    const float gmax = (float)(sc.G[0] - 1);
+    const uint32_t x0 = (uint32_t)(int)fminf(fmaxf(g.fx - wx, 0.0f), gmax);
+    const uint32_t x1 = (uint32_t)(int)fminf(fmaxf(g.fx + wx, 0.0f), gmax);
+    const uint32_t c = g.row0 + (uint32_t)(dz * sc.G[1] + dy) * (uint32_t)sc.G[0];
+    lo = __builtin_amdgcn_raw_buffer_load_b32(rcs, (int)((c + x0) << 2), 0, 0);
+    hi = __builtin_amdgcn_raw_buffer_load_b32(rcs, (int)((c + x1 + 1u) << 2), 0, 0);
+}
+
+// Row masks.  The density pass tests a row's candidates [lo, hi) four at a
+// time and shifts one bit per slot into the row's mask (slots past hi shift
+// in zeros), so candidate c of a row with T groups sits at bit 4T-1-c and
+// only the last 32 slots survive.  Rows of more than 32 candidates re-test
+// the ones before with the predicate.  row_neighbors visits a row's
+// neighbours in ascending order: first the re-tested head, then the bits
+// from the highest (the lowest candidate) down.
+__device__ __forceinline__ uint32_t mask_span(uint32_t lo, uint32_t hi) { return (hi - lo + 3u) & ~3u; }
+
+// Bit of candidate j of a row (>= 32 when j precedes the mask's slots).
+__device__ __forceinline__ uint32_t mask_bit(uint32_t lo, uint32_t hi, uint32_t j) {
+    return mask_span(lo, hi) - 1u - (j - lo);
+}
+
+template <class Retest, class F>
+__device__ __forceinline__ void row_neighbors(uint32_t lo, uint32_t hi, uint32_t m, Retest&& retest, F&& f) {
+    const uint32_t span = mask_span(lo, hi);
+    const uint32_t jb = lo + span - 32u;  // candidate of bit 31 (wraps when span < 32)
+    if (span > 32u)
+        for (uint32_t j = lo; j < jb; ++j)
+            if (retest(j)) f(j);
+    while (m) {
+        const uint32_t z = (uint32_t)__builtin_clz(m);
+        m ^= 0x80000000u >> z;
+        f(jb + z);
+    }
+}
+
 
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
@@ -159,13 +220,13 @@ __device__ __forceinline__ uint32_t cell_key(const SceneConst& sc, float x, floa
     return sc.cell_offset + ((uint32_t)cz * (uint32_t)sc.G[1] + (uint32_t)cy) * (uint32_t)sc.G[0] + (uint32_t)cx;
 }
 
-// Per-cell count with one atomic per run of equal keys in consecutive lanes
-// (lanes follow sorted order, so most neighbours share a cell).  Returns the
-// slot this lane took in its cell.  All lanes of the wave must call it;
+// Atomic add of each active lane's 1 to arr[idx], one atomic per run of
+// equal idx in consecutive lanes.  Returns the slot this lane took (the old
+// value plus its place in the run).  All lanes of the wave must call it;
 // `active` marks the lanes that hold a particle.
-__device__ __forceinline__ uint32_t count_slot(uint32_t* __restrict__ count, uint32_t key, bool active) {
+__device__ __forceinline__ uint32_t run_add(uint32_t* __restrict__ arr, uint32_t idx, bool active) {
     const int lane = (int)(threadIdx.x & 63u);
-    const uint32_t kk = active ? key : 0xffffffffu;
+    const uint32_t kk = active ? idx : 0xffffffffu;
     const uint32_t prev = __shfl_up(kk, 1, 64);
     const bool start = active && (lane == 0 || prev != kk);
     const uint64_t starts = __ballot(start);
@@ -177,9 +238,23 @@ __device__ __forceinline__ uint32_t count_slot(uint32_t* __restrict__ count, uin
     const uint64_t above = (starts | ~act) & ~((2ull << lane) - 1ull) & (lane == 63 ? 0ull : ~0ull);
     const int next = above ? __ffsll((long long)above) - 1 : 64;
     uint32_t base = 0;
-    if (start) base = atomicAdd(&count[kk], (uint32_t)(next - lane));
+    if (start) base = atomicAdd(&arr[kk], (uint32_t)(next - lane));
     base = __shfl(base, leader, 64);
     return base + (uint32_t)(lane - leader);
+}
+
+// Cells per tile of the cell-start scan (4096): the integrate pass also sums
+// particles per tile, so every scan block finds its prefix without waiting.
+constexpr uint32_t kScanTileShift = 12;
+
+// Per-cell count (lanes follow sorted order, so most neighbours share a
+// cell) and the per-tile sums that follow the C cell counts.  Returns the
+// slot this lane took in its cell.
+__device__ __forceinline__ uint32_t count_slot(uint32_t* __restrict__ count, uint32_t* __restrict__ tsum,
+                                               uint32_t key, bool active) {
+    const uint32_t slot = run_add(count, key, active);
+    (void)run_add(tsum, key >> kScanTileShift, active);
+    return slot;
 }
 
 // Tait EOS, exponent 7 by the chain x^2, x^4, (x^4 x^2) x; p < 0 is clamped

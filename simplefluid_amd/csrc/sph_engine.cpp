@@ -122,10 +122,17 @@ uint2* tiles_for(sph_handle* h, int block, uint32_t& ntiles) {
     return d.p;
 }
 
-uint32_t grid_for(sph_handle* h, int block, uint32_t ntiles) {
-    if (!h->persistent) return ntiles;
-    const uint32_t resident = (uint32_t)h->cu_count * (uint32_t)std::max(1, 2048 / block);
-    return std::min(ntiles, resident);
+// Persistent launches keep about 2048 lanes per CU resident and loop over
+// tile queues; they are used only when there are more tiles than that.
+static uint32_t resident_blocks(const sph_handle* h, int block) {
+    return (uint32_t)std::max(8, h->cu_count) * (uint32_t)std::max(1, 2048 / block);
+}
+
+uint32_t grid_for(sph_handle* h, int block, uint32_t ntiles, uint32_t** queue) {
+    if (queue) *queue = nullptr;
+    if (!h->tune.persistent || !queue || ntiles <= resident_blocks(h, block)) return ntiles;
+    *queue = h->queue.p;
+    return resident_blocks(h, block);
 }
 
 StepArgs step_args(sph_handle* h, StepBufs& w, int block) {
@@ -142,12 +149,12 @@ StepArgs step_args(sph_handle* h, StepBufs& w, int block) {
     a.count = h->count.p;
     a.cstart = w.cstart.p;
     a.rho = w.rho.p;
-    a.aux = w.aux.p;
     a.mask = w.mask.p;
     a.acc = (h->keep_acc || w.acc.p) ? w.acc.p : nullptr;
     a.scenes = h->sc_d.p;
     a.tiles = tiles_for(h, block, a.ntiles);
     a.n = h->n;
+    a.ncells = h->ncells;
     return a;
 }
 
@@ -167,7 +174,6 @@ int alloc_step_bufs(sph_handle* h, StepBufs& w) {
         for (int i = 0; i < 2; ++i) { chk(w.kbuf[i].ensure(n)); chk(w.vbuf[i].ensure(n)); }
     }
     chk(w.rho.ensure(n));
-    chk(w.aux.ensure(n));
     chk(w.mask.ensure((size_t)kRows * n));
     if (h->keep_acc) chk(w.acc.ensure(n));
     return hip_fail(h, e, "device allocation");
@@ -182,21 +188,21 @@ static int ensure_buffers(sph_handle* h) {
     chk(h->id.ensure(n));
     chk(h->key0.ensure(n));
     chk(h->rank0.ensure(n));
-    const bool fresh_count = h->count.count < h->ncells || !h->count.p;
-    chk(h->count.ensure(h->ncells));
+    const size_t count_words = (size_t)h->ncells + 2 * (size_t)scan_tiles(h->ncells);
+    const bool fresh_count = h->count.count < count_words || !h->count.p;
+    chk(h->count.ensure(count_words));
     chk(h->hist.ensure(kMaxPasses * 256));
     chk(h->status.ensure((size_t)kMaxPasses * sort_tiles((uint32_t)n, 4) * 256 + 256));
     chk(h->tickets.ensure(kMaxPasses));
-    const bool fresh_scan = h->scan_status.count < scan_tiles(h->ncells) + 1 || !h->scan_status.p;
-    chk(h->scan_status.ensure(scan_tiles(h->ncells) + 1));
     chk(h->err.ensure(1));
+    chk(h->queue.ensure(kQueueWords));
     chk(h->staging.ensure(3 * n + 1));
     chk(h->sc_d.ensure(h->sc_h.size()));
     if (e != hipSuccess) return hip_fail(h, e, "device allocation");
     chk(launch_fill_u32(h->stream, h->hist.p, 0u, kMaxPasses * 256));
     chk(launch_fill_u32(h->stream, h->err.p, 0u, 1));
-    if (fresh_count) chk(launch_fill_u32(h->stream, h->count.p, 0u, h->ncells));
-    if (fresh_scan) chk(launch_fill_u32(h->stream, h->scan_status.p, 0u, scan_tiles(h->ncells) + 1));
+    chk(launch_fill_u32(h->stream, h->queue.p, 0u, kQueueWords));
+    if (fresh_count) chk(launch_fill_u32(h->stream, h->count.p, 0u, count_words));
     chk(hipMemcpyAsync(h->sc_d.p, h->sc_h.data(), h->sc_h.size() * sizeof(SceneConst), hipMemcpyHostToDevice, h->stream));
     if (e != hipSuccess) return hip_fail(h, e, "buffer init");
     return alloc_step_bufs(h, h->work);
@@ -217,11 +223,12 @@ int enqueue_step(sph_handle* h, StepBufs& w, bool record_prof) {
     if (record_prof) prof_record(h, 0);
     if (!h->keys_valid) {
         // New state: counts start from zero, then every particle is hashed.
-        hipError_t e = launch_fill_u32(st, h->count.p, 0u, h->ncells);
+        hipError_t e = launch_fill_u32(st, h->count.p, 0u, (size_t)h->ncells + 2 * (size_t)scan_tiles(h->ncells));
         uint32_t nt = 0;
         const uint2* tl = tiles_for(h, 256, nt);
         if (e == hipSuccess)
-            e = launch_hash(st, 256, nt, h->pos.p, h->key0.p, h->rank0.p, h->count.p, h->sc_d.p, tl, nt);
+            e = launch_hash(st, 256, nt, h->pos.p, h->key0.p, h->rank0.p, h->count.p, h->ncells, h->sc_d.p, tl,
+                            nt);
         if (e != hipSuccess) return hip_fail(h, e, "hash launch");
         h->keys_valid = true;
     }
@@ -244,8 +251,8 @@ int enqueue_step(sph_handle* h, StepBufs& w, bool record_prof) {
     sp.hist = h->hist.p;
     sp.status = h->status.p;
     sp.tickets = h->tickets.p;
-    sp.scan_status = h->scan_status.p;
     sp.err = h->err.p;
+    sp.queue = h->queue.p;
     sp.pos_in = h->pos.p;
     sp.vel_in = h->vel.p;
     sp.id_in = h->id.p;
@@ -260,13 +267,15 @@ int enqueue_step(sph_handle* h, StepBufs& w, bool record_prof) {
     if (record_prof) prof_record(h, 3);
     {
         StepArgs a = step_args(h, w, h->tune.density_block);
-        e = launch_density(st, h->tune.density_block, grid_for(h, h->tune.density_block, a.ntiles), a);
+        const uint32_t grid = grid_for(h, h->tune.density_block, a.ntiles, &a.queue);
+        e = launch_density(st, h->tune.density_block, grid, a);
         if (e != hipSuccess) return hip_fail(h, e, "density launch");
     }
     if (record_prof) prof_record(h, 4);
     {
         StepArgs a = step_args(h, w, h->tune.force_block);
-        e = launch_force(st, h->tune.force_block, grid_for(h, h->tune.force_block, a.ntiles), a);
+        const uint32_t grid = grid_for(h, h->tune.force_block, a.ntiles, &a.queue);
+        e = launch_force(st, h->tune.force_block, grid, a);
         if (e != hipSuccess) return hip_fail(h, e, "force launch");
     }
     if (record_prof) {
@@ -315,7 +324,9 @@ static int pick_xsub(const std::vector<sph_params>& ps, uint64_t n, int override
         if (derive_scene(p, sc, err, 1) != SPH_OK) return 1;
         cells1 += (uint64_t)sc.G[0] * sc.G[1] * sc.G[2];
     }
-    const uint64_t budget = std::max<uint64_t>(4 * n, 1u << 20);
+    // Finer x cells shorten the neighbour rows; the cost is a scan over C
+    // cells per step.  Up to ~12 cells per particle the shorter rows win.
+    const uint64_t budget = std::max<uint64_t>(12 * n, 1u << 20);
     for (int x : {4, 2}) if (cells1 * (uint64_t)x <= budget) return x;
     return 1;
 }
@@ -632,7 +643,6 @@ int sph_batch_create(const sph_scene_desc* scenes, uint32_t nscenes, sph_handle*
     std::vector<uint32_t> ns(nscenes);
     for (uint32_t s = 0; s < nscenes; ++s) { ps[s] = scenes[s].params; ns[s] = scenes[s].num_particles; }
     h->batched = true;
-    h->persistent = true;
     rc = configure(h, ps, ns);
     if (rc == SPH_OK) rc = ensure_buffers(h);
     if (rc != SPH_OK) {
@@ -709,6 +719,7 @@ int sph_get_tuning(const sph_handle* h, int* cfg) {
     cfg[1] = h->tune.force_block;
     cfg[2] = h->tune.reorder_block;
     cfg[3] = h->tune.sort_items;
+    cfg[4] = h->tune.persistent;
     return SPH_OK;
 }
 
@@ -720,10 +731,12 @@ int sph_set_tuning(sph_handle* h, const int* cfg) {
         return fail(h, SPH_E_INVALID, "block sizes must be 64, 128, 256 or 512");
     if (cfg[3] != 4 && cfg[3] != 8 && cfg[3] != 12 && cfg[3] != 16)
         return fail(h, SPH_E_INVALID, "sort items must be 4, 8, 12 or 16");
+    if (cfg[4] != 0 && cfg[4] != 1) return fail(h, SPH_E_INVALID, "persistent must be 0 or 1");
     h->tune.density_block = cfg[0];
     h->tune.force_block = cfg[1];
     h->tune.reorder_block = cfg[2];
     h->tune.sort_items = cfg[3];
+    h->tune.persistent = cfg[4];
     invalidate_graph(h);
     return SPH_OK;
 }

@@ -43,7 +43,6 @@ struct StepBufs {
     DevBuf<uint32_t> tmp;          // cell-count sort slots
     DevBuf<uint32_t> kbuf[2], vbuf[2];  // radix ping-pong (radix mode only)
     DevBuf<float> rho;
-    DevBuf<float2> aux;
     DevBuf<uint32_t> mask;
     DevBuf<float4> acc;
 };
@@ -53,6 +52,7 @@ struct Tuning {
     int force_block = 256;
     int reorder_block = 256;
     int sort_items = 8;
+    int persistent = 0;   // density and force: 0 one block per tile, 1 persistent blocks over tile queues
     int sort_mode = kSortCellCount;
 };
 
@@ -93,7 +93,6 @@ struct sph_handle {
     bool has_step = false;
     bool keys_valid = false;
     bool keep_acc = false;
-    bool persistent = false;
     int cu_count = 256;
     std::string arch;
     sph::Tuning tune;
@@ -103,7 +102,7 @@ struct sph_handle {
     sph::DevBuf<float4> pos, vel;
     sph::DevBuf<uint32_t> id, key0, rank0, count;
     sph::StepBufs work;
-    sph::DevBuf<uint32_t> hist, status, tickets, scan_status, err;
+    sph::DevBuf<uint32_t> hist, status, tickets, err, queue;
     sph::DevBuf<sph::SceneConst> sc_d;
     std::map<int, sph::DevBuf<uint2>> tiles;
     std::map<int, uint32_t> ntiles;
@@ -135,7 +134,9 @@ int derive_scene(const sph_params& p, SceneConst& sc, std::string& err, int xsub
 int alloc_step_bufs(sph_handle* h, StepBufs& w);
 void invalidate_graph(sph_handle* h);
 uint2* tiles_for(sph_handle* h, int block, uint32_t& ntiles);
-uint32_t grid_for(sph_handle* h, int block, uint32_t ntiles);
+// Grid of a density or force launch; *queue receives the tile counters when
+// the launch is persistent, else null.
+uint32_t grid_for(sph_handle* h, int block, uint32_t ntiles, uint32_t** queue);
 StepArgs step_args(sph_handle* h, StepBufs& w, int block);
 // Enqueue one step reading/writing the state buffers with intermediates in w.
 int enqueue_step(sph_handle* h, StepBufs& w, bool record_prof);

@@ -22,7 +22,7 @@ struct SortPlan {
     int reorder_block = 256;
     const uint32_t* key0 = nullptr;   // unsorted keys (state order)
     const uint32_t* rank0 = nullptr;  // slot each particle took in its cell's count
-    uint32_t* count = nullptr;        // C, zeroed by the scan
+    uint32_t* count = nullptr;        // C cell counts (zeroed by the scan) + per-tile sums
     uint32_t* cstart = nullptr;       // C + 1, written by the scan
     uint32_t* tmp = nullptr;          // cell-count: slot -> particle
     uint32_t* kbuf[2] = {nullptr, nullptr};  // radix ping-pong
@@ -32,8 +32,8 @@ struct SortPlan {
     uint32_t* hist = nullptr;         // radix: kMaxPasses x 256, zero on entry
     uint32_t* status = nullptr;       // radix: passes x tiles x 256
     uint32_t* tickets = nullptr;      // radix: kMaxPasses
-    uint32_t* scan_status = nullptr;  // scan tiles + 1 (ticket), zero on entry
     uint32_t* err = nullptr;
+    uint32_t* queue = nullptr;        // kQueueWords tile counters, zeroed by the scan
     const float4* pos_in = nullptr;
     const float4* vel_in = nullptr;
     const uint32_t* id_in = nullptr;
@@ -43,6 +43,8 @@ struct SortPlan {
 };
 
 uint32_t sort_tiles(uint32_t n, int items);
+// Tiles of 4096 cells.  count holds C cell counts, then T = scan_tiles(C)
+// per-tile sums, then their exclusive scan (T words).
 uint32_t scan_tiles(uint32_t ncells);
 int sort_key_passes(uint32_t ncells);
 // Cell-start table plus the sort proper (keys and permutation) ...
@@ -53,29 +55,34 @@ hipError_t launch_sort_gather(hipStream_t st, const SortPlan& sp);
 // Arguments of the per-particle passes.  Work is split in tiles of blockDim
 // particles that never straddle two scenes (tiles[t] = (scene, first)).
 struct StepArgs {
-    const float4* pos_s;     // sorted positions
-    const float4* vel_s;     // sorted velocities
+    float4* pos_s;           // sorted positions; w <- p/rho^2 (density pass)
+    float4* vel_s;           // sorted velocities; w <- 1/rho (density pass)
     float4* pos_o;           // state out (integrate)
     float4* vel_o;
     const uint32_t* id_s;
     uint32_t* id_o;
     uint32_t* key0;          // next step's unsorted keys
     uint32_t* rank0;         // next step's slot in its cell count
-    uint32_t* count;         // next step's per-cell counts (atomic)
+    uint32_t* count;         // next step's per-cell counts, then per-scan-tile sums (atomic)
     const uint32_t* cstart;  // cell-start table of this step
     float* rho;
-    float2* aux;
     uint32_t* mask;
     float4* acc;             // optional: acceleration (sorted order)
     const SceneConst* scenes;
     const uint2* tiles;
     uint32_t ntiles;
     uint32_t n;
+    uint32_t ncells;
+    uint32_t* queue;         // persistent launch: 8 tile counters (zeroed by the scan), else null
 };
 
+// Tile counters of the persistent density and force launches (8 each),
+// zeroed by the cell-start scan of every step.
+constexpr int kQueueWords = 16;
+
 hipError_t launch_hash(hipStream_t st, int block, uint32_t grid, const float4* pos, uint32_t* key0,
-                       uint32_t* rank0, uint32_t* count, const SceneConst* scenes, const uint2* tiles,
-                       uint32_t ntiles);
+                       uint32_t* rank0, uint32_t* count, uint32_t ncells, const SceneConst* scenes,
+                       const uint2* tiles, uint32_t ntiles);
 hipError_t launch_density(hipStream_t st, int block, uint32_t grid, const StepArgs& a);
 hipError_t launch_force(hipStream_t st, int block, uint32_t grid, const StepArgs& a);
 hipError_t launch_fill_u32(hipStream_t st, uint32_t* p, uint32_t value, size_t count);
@@ -96,7 +103,6 @@ struct AdjArgs {
     const float4* vel_s;
     const float4* acc;
     const float* rho;
-    const float2* aux;
     const uint32_t* mask;
     const uint32_t* cstart;
     const uint32_t* perm;
@@ -112,6 +118,7 @@ struct AdjArgs {
     const uint2* tiles;
     uint32_t ntiles;
     uint32_t n;
+    uint32_t ncells;
 };
 
 hipError_t launch_adj_integrate(hipStream_t st, int block, uint32_t grid, const AdjArgs& a);

@@ -99,7 +99,7 @@ int sph_rollout_backward(sph_handle* h, const float* dL_dx_final, float* dL_dB, 
 
     uint32_t ntiles = 0;
     const uint2* tiles = tiles_for(h, 256, ntiles);
-    const uint32_t grid = grid_for(h, 256, ntiles);
+    const uint32_t grid = ntiles;  // one block per tile
     for (int seg = (int)R.nseg - 1; seg >= 0; --seg) {
         chk(load_checkpoint(h, R, (uint32_t)seg));
         if (e != hipSuccess) return hip_fail(h, e, "checkpoint restore");
@@ -112,11 +112,11 @@ int sph_rollout_backward(sph_handle* h, const float* dL_dx_final, float* dL_dB, 
             StepBufs& w = *R.tape[t];
             AdjArgs a;
             std::memset(&a, 0, sizeof(a));
-            a.pos_s = w.pos_s.p; a.vel_s = w.vel_s.p; a.acc = w.acc.p; a.rho = w.rho.p; a.aux = w.aux.p;
+            a.pos_s = w.pos_s.p; a.vel_s = w.vel_s.p; a.acc = w.acc.p; a.rho = w.rho.p;
             a.mask = w.mask.p; a.cstart = w.cstart.p; a.perm = w.perm.p;
             a.xbar = R.xbar.p; a.vbar = R.vbar.p; a.xs = R.xs.p; a.vs = R.vs.p; a.abar = R.abar.p;
             a.rhobar = R.rhobar.p; a.bpart = R.bpart.p; a.mpart = R.mpart.p;
-            a.scenes = h->sc_d.p; a.tiles = tiles; a.ntiles = ntiles; a.n = (uint32_t)n;
+            a.scenes = h->sc_d.p; a.tiles = tiles; a.ntiles = ntiles; a.n = (uint32_t)n; a.ncells = h->ncells;
             chk(launch_adj_integrate(st, 256, grid, a));
             chk(launch_adj_force(st, 256, grid, a));
             chk(launch_scene_reduce(st, ns, h->sc_d.p, R.bpart.p, R.gB.p));

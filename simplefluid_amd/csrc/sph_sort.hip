@@ -6,8 +6,9 @@
 // Cell-count sort (default): a one-digit radix sort whose digit is the whole
 // cell key.  The integrate pass already counted particles per cell with
 // atomics (count[key]++, keeping the returned slot in rank0).  Then
-//   cell_scan        exclusive scan of count -> cstart (decoupled look-back,
-//                    zeroes count for the next step)
+//   cell_scan        exclusive scan of count -> cstart; each block of 4096
+//                    cells takes its prefix from the per-tile sums the
+//                    integrate pass also counted (zeroes count for the next step)
 //   cell_scatter     tmp[cstart[key] + rank0] = k   (slot order within a
 //                    cell is whatever order the atomics ran in)
 //   cell_rank        the deterministic rank of k in its cell is the number of
@@ -36,8 +37,6 @@ constexpr uint32_t kFlagInc = 2u << 30;
 constexpr uint32_t kValMask = (1u << 30) - 1u;
 constexpr uint32_t kSpinLimit = 1u << 22;
 constexpr int kHistItems = 16;
-constexpr int kScanItems = 32;
-constexpr int kScanTile = kSortBlock * kScanItems;
 
 __device__ __forceinline__ uint32_t ld_agent(const uint32_t* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -94,94 +93,141 @@ __device__ __forceinline__ uint32_t look_back(const uint32_t* status, uint32_t s
     return prefix;
 }
 
-// Wave-wide look-back for one value per tile: lane l probes tile
-// (tile - 1 - l) of the current 64-tile window; the closest inclusive prefix
-// ends the walk, otherwise all 64 aggregates are added and the window moves
-// back.  Tiles before 0 read as an inclusive 0.  Called by all 64 lanes of
-// one wave; returns the exclusive prefix of `tile`.
-__device__ __forceinline__ uint32_t wave_look_back(const uint32_t* status, uint32_t tile, uint32_t* err) {
-    const int lane = threadIdx.x & 63;
-    uint32_t prefix = 0;
-    int base = (int)tile - 1;
-    uint32_t spins = 0;
-    while (true) {
-        const int t = base - lane;
-        const uint32_t s = t >= 0 ? ld_agent(status + t) : kFlagInc;
-        if (__any((s & ~kValMask) == 0u)) {
-            if (++spins > kSpinLimit) {
-                if (lane == 0) atomicOr(err, 1u);
-                break;
-            }
-            __builtin_amdgcn_s_sleep(1);
-            continue;
-        }
-        const uint64_t inc = __ballot((s & ~kValMask) == kFlagInc);
-        const int first = inc ? __ffsll((long long)inc) - 1 : 64;
-        uint32_t v = lane <= first ? (s & kValMask) : 0u;
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-        prefix += v;
-        if (inc) break;
-        base -= 64;
-    }
-    return prefix;
-}
-
 // ---------------------------------------------------------------- cell count
 
+// Exclusive scan of the T per-tile sums (count[C .. C+T)) into
+// count[C+T .. C+2T), one block of 1024 threads, each over a contiguous run.
+constexpr int kTileScanBlock = 1024;
+constexpr uint32_t kInlineTiles = 1024;
+
+__global__ __launch_bounds__(kTileScanBlock) void tile_scan_kernel(uint32_t* __restrict__ count, uint32_t C,
+                                                                   uint32_t T) {
+    __shared__ uint32_t s_w[kTileScanBlock / 64];
+    const uint32_t* tsum = count + C;
+    uint32_t* tpre = count + C + T;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t per = (T + kTileScanBlock - 1) / kTileScanBlock;
+    const uint32_t i0 = (uint32_t)tid * per;
+    const uint32_t i1 = i0 + per < T ? i0 + per : T;
+    uint32_t own = 0;
+    for (uint32_t i = i0; i < i1; ++i) own += tsum[i];
+    uint32_t x = own;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) s_w[wave] = x;
+    __syncthreads();
+    uint32_t run = x - own;
+    for (int w = 0; w < wave; ++w) run += s_w[w];
+    for (uint32_t i = i0; i < i1; ++i) {
+        tpre[i] = run;
+        run += tsum[i];
+    }
+}
+
+// Block b scans the counts of cells [4096 b, 4096 (b + 1)).  Its prefix is
+// the exclusive scan of the tile sums (counted by the integrate pass next to
+// the cell counts): summed here when there are at most kInlineTiles tiles,
+// else read from tile_scan_kernel's output.  No block waits for another; a
+// tile with no particles only writes its prefix.  Wave w takes 1024 cells in 4 rounds of 64 lanes x
+// 4 consecutive cells (16-byte loads and stores); the counts are zeroed for
+// the next step as they are read.
 __global__ __launch_bounds__(kSortBlock) void cell_scan_kernel(uint32_t* __restrict__ count,
                                                                uint32_t* __restrict__ cstart, uint32_t C,
-                                                               uint32_t* status, uint32_t* ticket, uint32_t* err) {
-    __shared__ uint32_t s_tile, s_prefix;
-    __shared__ uint32_t s_w[kSortWaves];
-    const int tid = threadIdx.x;
-    if (tid == 0) s_tile = atomicAdd(ticket, 1u);
-    __syncthreads();
-    const uint32_t tile = s_tile;
-    const uint32_t first = tile * (uint32_t)kScanTile + (uint32_t)tid * kScanItems;
-    uint32_t v[kScanItems];
-    uint32_t sum = 0;
-    if (first + kScanItems <= C) {
-        const uint4* src = reinterpret_cast<const uint4*>(count + first);
+                                                               uint32_t* __restrict__ queue) {
+    __shared__ uint32_t s_tot[kSortWaves];
+    const uint32_t b = blockIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // Tile counters of this step's persistent density and force launches.
+    if (b == 0 && tid < kQueueWords && queue) queue[tid] = 0u;
+    const uint32_t T = (C + (1u << kScanTileShift) - 1u) >> kScanTileShift;
+    uint32_t pre;
+    if (T <= kInlineTiles) {
+        __shared__ uint32_t s_pre[kSortWaves];
+        uint32_t x = 0;
+        for (uint32_t i = (uint32_t)tid; i < b; i += kSortBlock) x += count[C + i];
 #pragma unroll
-        for (int q = 0; q < kScanItems / 4; ++q) {
-            const uint4 u = src[q];
-            v[4 * q] = u.x; v[4 * q + 1] = u.y; v[4 * q + 2] = u.z; v[4 * q + 3] = u.w;
-        }
-        uint4* dst = reinterpret_cast<uint4*>(count + first);
-#pragma unroll
-        for (int q = 0; q < kScanItems / 4; ++q) dst[q] = make_uint4(0u, 0u, 0u, 0u);
+        for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+        if (lane == 0) s_pre[wave] = x;
+        __syncthreads();
+        pre = s_pre[0] + s_pre[1] + s_pre[2] + s_pre[3];
     } else {
-#pragma unroll
-        for (int i = 0; i < kScanItems; ++i) {
-            const uint32_t idx = first + i;
-            v[i] = idx < C ? count[idx] : 0u;
-            if (idx < C) count[idx] = 0u;
-        }
+        pre = count[C + T + b];
     }
+    if (count[C + b] == 0u) {
+        const uint32_t t0 = b << kScanTileShift;
+        const uint32_t t1 = t0 + (1u << kScanTileShift) < C ? t0 + (1u << kScanTileShift) : C;
+        const uint4 v4 = make_uint4(pre, pre, pre, pre);
+        for (uint32_t i = t0 + 4u * (uint32_t)tid; i < t1; i += 4u * kSortBlock) {
+            if (i + 4u <= t1) {
+                *reinterpret_cast<uint4*>(cstart + i) = v4;
+            } else {
+                for (uint32_t e = i; e < t1; ++e) cstart[e] = pre;
+            }
+        }
+        if (t1 == C && tid == 0) cstart[C] = pre;
+        return;
+    }
+    const uint32_t base = (b << kScanTileShift) + (uint32_t)wave * 1024u;
+    uint4 v[4];
 #pragma unroll
-    for (int i = 0; i < kScanItems; ++i) sum += v[i];
-    uint32_t total = 0;
-    const uint32_t excl = block_exclusive_scan(sum, s_w, &total);
-    if (tid < 64) {
-        uint32_t prefix = 0;
-        if (tile == 0) {
-            if (tid == 0) st_agent(status, kFlagInc | total);
+    for (int r = 0; r < 4; ++r) {
+        const uint32_t i0 = base + 256u * (uint32_t)r + 4u * (uint32_t)lane;
+        if (i0 + 4u <= C) {
+            v[r] = *reinterpret_cast<const uint4*>(count + i0);
+            *reinterpret_cast<uint4*>(count + i0) = make_uint4(0u, 0u, 0u, 0u);
         } else {
-            if (tid == 0) st_agent(status + tile, kFlagAgg | total);
-            prefix = wave_look_back(status, tile, err);
-            if (tid == 0) st_agent(status + tile, kFlagInc | (prefix + total));
+            v[r].x = i0 < C ? count[i0] : 0u;
+            v[r].y = i0 + 1u < C ? count[i0 + 1u] : 0u;
+            v[r].z = i0 + 2u < C ? count[i0 + 2u] : 0u;
+            v[r].w = i0 + 3u < C ? count[i0 + 3u] : 0u;
+            for (uint32_t e = 0; e < 4u; ++e)
+                if (i0 + e < C) count[i0 + e] = 0u;
         }
-        if (tid == 0) s_prefix = prefix;
     }
-    __syncthreads();
-    uint32_t run = s_prefix + excl;
+    // Wave-local exclusive prefix of each lane's four cells, round by round.
+    uint32_t ex[4];
+    uint32_t carry = 0;
 #pragma unroll
-    for (int i = 0; i < kScanItems; ++i) {
-        const uint32_t idx = first + i;
-        if (idx < C) cstart[idx] = run;
-        run += v[i];
-        if (idx + 1 == C) cstart[C] = run;
+    for (int r = 0; r < 4; ++r) {
+        const uint32_t own = v[r].x + v[r].y + v[r].z + v[r].w;
+        uint32_t x = own;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(x, o, 64);
+            if (lane >= o) x += y;
+        }
+        ex[r] = carry + x - own;
+        carry += __shfl(x, 63, 64);
+    }
+    if (lane == 0) s_tot[wave] = carry;
+    __syncthreads();
+    uint32_t off = pre;
+#pragma unroll
+    for (int w = 0; w < kSortWaves; ++w)
+        if (w < wave) off += s_tot[w];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const uint32_t i0 = base + 256u * (uint32_t)r + 4u * (uint32_t)lane;
+        uint4 o;
+        o.x = off + ex[r];
+        o.y = o.x + v[r].x;
+        o.z = o.y + v[r].y;
+        o.w = o.z + v[r].z;
+        const uint32_t end = o.w + v[r].w;
+        if (i0 + 4u <= C) {
+            *reinterpret_cast<uint4*>(cstart + i0) = o;
+        } else {
+            const uint32_t val[4] = {o.x, o.y, o.z, o.w};
+            for (uint32_t e = 0; e < 4u; ++e)
+                if (i0 + e < C) cstart[i0 + e] = val[e];
+        }
+        if (i0 < C && C <= i0 + 4u) {
+            const uint32_t val[5] = {o.x, o.y, o.z, o.w, end};
+            cstart[C] = val[C - i0];
+        }
     }
 }
 
@@ -204,10 +250,11 @@ __global__ void cell_rank_kernel(uint32_t n, uint32_t C, const uint32_t* __restr
                                  const uint32_t* __restrict__ id_in, float4* __restrict__ pos_out,
                                  float4* __restrict__ vel_out, uint32_t* __restrict__ id_out,
                                  uint32_t* __restrict__ skey, uint32_t* __restrict__ perm,
-                                 uint32_t* __restrict__ scan_reset, uint32_t scan_reset_words,
+                                 uint32_t* __restrict__ tsum, uint32_t tsum_words,
                                  uint32_t* __restrict__ err) {
+    // The scan has read the tile sums; clear them for the integrate pass.
     if (blockIdx.x == 0)
-        for (uint32_t i = threadIdx.x; i < scan_reset_words; i += blockDim.x) scan_reset[i] = 0u;
+        for (uint32_t i = threadIdx.x; i < tsum_words; i += blockDim.x) tsum[i] = 0u;
     const uint32_t k = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const uint32_t key = key0[k];
@@ -384,11 +431,11 @@ __global__ void radix_reorder_kernel(uint32_t n, const uint32_t* __restrict__ pe
                                      const float4* __restrict__ pos_in, const float4* __restrict__ vel_in,
                                      const uint32_t* __restrict__ id_in, float4* __restrict__ pos_out,
                                      float4* __restrict__ vel_out, uint32_t* __restrict__ id_out,
-                                     uint32_t* __restrict__ hist_zero, uint32_t* __restrict__ scan_reset,
-                                     uint32_t scan_reset_words, uint32_t* __restrict__ err) {
+                                     uint32_t* __restrict__ hist_zero, uint32_t* __restrict__ tsum,
+                                     uint32_t tsum_words, uint32_t* __restrict__ err) {
     if (blockIdx.x == 0) {
         for (uint32_t i = threadIdx.x; i < (uint32_t)(kMaxPasses * kBins); i += blockDim.x) hist_zero[i] = 0u;
-        for (uint32_t i = threadIdx.x; i < scan_reset_words; i += blockDim.x) scan_reset[i] = 0u;
+        for (uint32_t i = threadIdx.x; i < tsum_words; i += blockDim.x) tsum[i] = 0u;
     }
     const uint32_t k = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (k >= n) return;
@@ -411,7 +458,7 @@ uint32_t sort_tiles(uint32_t n, int items) {
     return (n + tile - 1) / tile;
 }
 
-uint32_t scan_tiles(uint32_t C) { return (C + kScanTile - 1) / kScanTile; }
+uint32_t scan_tiles(uint32_t C) { return (C + (1u << kScanTileShift) - 1u) >> kScanTileShift; }
 
 int sort_key_passes(uint32_t ncells) {
     int bits = 1;
@@ -423,8 +470,10 @@ hipError_t launch_sort_keys(hipStream_t st, const SortPlan& sp) {
     if (sp.n == 0) return hipSuccess;
     // The cell-start table (both modes).
     const uint32_t stiles = scan_tiles(sp.ncells);
+    if (stiles > kInlineTiles)
+        hipLaunchKernelGGL(tile_scan_kernel, dim3(1), dim3(kTileScanBlock), 0, st, sp.count, sp.ncells, stiles);
     hipLaunchKernelGGL(cell_scan_kernel, dim3(stiles), dim3(kSortBlock), 0, st, sp.count, sp.cstart, sp.ncells,
-                       sp.scan_status, sp.scan_status + stiles, sp.err);
+                       sp.queue);
     if (sp.mode == kSortCellCount) {
         hipLaunchKernelGGL(cell_scatter_kernel, dim3(blocks_for(sp.n, 256)), dim3(256), 0, st, sp.n, sp.ncells,
                            sp.key0, sp.rank0, sp.cstart, sp.tmp, sp.err);
@@ -467,16 +516,17 @@ hipError_t launch_sort_keys(hipStream_t st, const SortPlan& sp) {
 
 hipError_t launch_sort_gather(hipStream_t st, const SortPlan& sp) {
     if (sp.n == 0) return hipSuccess;
-    const uint32_t scan_reset_words = scan_tiles(sp.ncells) + 1;  // status words + ticket
+    uint32_t* tsum = sp.count + sp.ncells;  // per-tile sums, cleared once the scan has read them
+    const uint32_t tsum_words = scan_tiles(sp.ncells);
     if (sp.mode == kSortCellCount)
         hipLaunchKernelGGL(cell_rank_kernel, dim3(blocks_for(sp.n, sp.reorder_block)), dim3(sp.reorder_block), 0,
                            st, sp.n, sp.ncells, sp.key0, sp.cstart, sp.tmp, sp.pos_in, sp.vel_in, sp.id_in,
-                           sp.pos_out, sp.vel_out, sp.id_out, sp.skey, sp.perm, sp.scan_status,
-                           scan_reset_words, sp.err);
+                           sp.pos_out, sp.vel_out, sp.id_out, sp.skey, sp.perm, tsum,
+                           tsum_words, sp.err);
     else
         hipLaunchKernelGGL(radix_reorder_kernel, dim3(blocks_for(sp.n, sp.reorder_block)), dim3(sp.reorder_block),
                            0, st, sp.n, sp.perm, sp.pos_in, sp.vel_in, sp.id_in, sp.pos_out, sp.vel_out,
-                           sp.id_out, sp.hist, sp.scan_status, scan_reset_words, sp.err);
+                           sp.id_out, sp.hist, tsum, tsum_words, sp.err);
     return hipGetLastError();
 }
 

@@ -23,12 +23,12 @@ __device__ __forceinline__ void tile_span(uint32_t ntiles, uint32_t& t0, uint32_
 }
 
 __device__ __forceinline__ void density_one(const StepArgs& a, const SceneConst& sc, uint32_t k,
-                                            __amdgpu_buffer_rsrc_t rpos) {
-    const float4 pi = a.pos_s[k];
+                                            __amdgpu_buffer_rsrc_t rpos, __amdgpu_buffer_rsrc_t rcs) {
+    const float4 pi = load4(rpos, k << 4);
     const RowGeom g = row_geom(sc, pi.x, pi.y, pi.z);
     uint32_t lo[kRows], hi[kRows];
 #pragma unroll
-    for (int row = 0; row < kRows; ++row) row_window(a.cstart, sc, g, row % 3 - 1, row / 3 - 1, lo[row], hi[row]);
+    for (int row = 0; row < kRows; ++row) row_window(rcs, sc, g, row % 3 - 1, row / 3 - 1, lo[row], hi[row]);
     const float h2 = sc.h2;
     float sum = 0.0f;
 #pragma unroll
@@ -36,7 +36,7 @@ __device__ __forceinline__ void density_one(const StepArgs& a, const SceneConst&
         uint32_t m = 0u;
         const uint32_t l = lo[row], e = hi[row];
         // Four candidates per trip, loads in flight together; slots past the
-        // row end are masked (past the array end the buffer returns 0).
+        // row end shift in zero bits (past the array end the buffer returns 0).
         for (uint32_t j = l; j < e; j += 4) {
             const uint32_t rem = e - j;
             const uint32_t off = j << 4;
@@ -46,11 +46,10 @@ __device__ __forceinline__ void density_one(const StepArgs& a, const SceneConst&
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const float d2 = dist2(pi.x - pj[u].x, pi.y - pj[u].y, pi.z - pj[u].z);
-                const bool hit = ((uint32_t)u < rem) && d2 < h2;
+                const bool hit = (u == 0 || (uint32_t)u < rem) && d2 < h2;
                 const float t = hit ? h2 - d2 : 0.0f;
                 sum = fmaf(t * t, t, sum);
-                const uint32_t b = j - l + (uint32_t)u;
-                if (hit && b < (uint32_t)kMaskBits) m |= 1u << b;
+                m = m + m + (hit ? 1u : 0u);
             }
         }
         a.mask[(size_t)row * a.n + k] = m;
@@ -59,28 +58,34 @@ __device__ __forceinline__ void density_one(const StepArgs& a, const SceneConst&
     const float p = eos(sc, rho);
     const float ir = 1.0f / rho;
     a.rho[k] = rho;
-    a.aux[k] = make_float2((p * ir) * ir, ir);
+    // p/rho^2 and 1/rho ride in the unused w of the sorted position and
+    // velocity, so the force pass gets them with the same loads.  Neighbour
+    // reads of pos_s in this pass take x, y, z only.
+    reinterpret_cast<float*>(a.pos_s + k)[3] = (p * ir) * ir;
+    reinterpret_cast<float*>(a.vel_s + k)[3] = ir;
 }
 
 struct ForceAcc {
     float px, py, pz, vx, vy, vz;
 };
 
-__device__ __forceinline__ void force_pair(const SceneConst& sc, const float4& pi, const float4& vi, float pti,
-                                           const float4& pj, const float4& vj, const float2& xj, ForceAcc& f) {
+// pos.w = p/rho^2 and vel.w = 1/rho of each particle.
+__device__ __forceinline__ void force_pair(const SceneConst& sc, const float4& pi, const float4& vi,
+                                           const float4& pj, const float4& vj, ForceAcc& f) {
     const float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
     const float d2 = dist2(dx, dy, dz);
-    // Hardware reciprocal square root: d2 of two distinct particles is never
-    // denormal, so the denormal rescaling of rsqrtf is not needed.
-    const float inv_r = d2 > 0.0f ? __builtin_amdgcn_rsqf(d2) : 0.0f;
+    // Hardware reciprocal square root.  d2 of two distinct particles is never
+    // denormal; coincident particles (d2 = 0) get a finite 1/r times a zero
+    // offset, so their pressure term vanishes as it does with 1/r := 0.
+    const float inv_r = __builtin_amdgcn_rsqf(fmaxf(d2, 1e-30f));
     const float r = d2 * inv_r;
     const float hr = sc.h - r;
     const float w = hr * hr;
-    const float s = ((pti + xj.x) * w) * inv_r;
+    const float s = ((pi.w + pj.w) * w) * inv_r;
     f.px = fmaf(s, dx, f.px);
     f.py = fmaf(s, dy, f.py);
     f.pz = fmaf(s, dz, f.pz);
-    const float vv = hr * xj.y;
+    const float vv = hr * vj.w;
     f.vx = fmaf(vv, vj.x - vi.x, f.vx);
     f.vy = fmaf(vv, vj.y - vi.y, f.vy);
     f.vz = fmaf(vv, vj.z - vi.z, f.vz);
@@ -88,41 +93,34 @@ __device__ __forceinline__ void force_pair(const SceneConst& sc, const float4& p
 
 __device__ __forceinline__ uint32_t force_one(const StepArgs& a, const SceneConst& sc, uint32_t k,
                                               __amdgpu_buffer_rsrc_t rpos, __amdgpu_buffer_rsrc_t rvel,
-                                              __amdgpu_buffer_rsrc_t raux) {
-    const float4 pi = a.pos_s[k];
-    const float4 vi = a.vel_s[k];
-    const float2 xi = a.aux[k];
+                                              __amdgpu_buffer_rsrc_t rcs) {
+    const float4 pi = load4(rpos, k << 4);
+    const float4 vi = load4(rvel, k << 4);
     const RowGeom g = row_geom(sc, pi.x, pi.y, pi.z);
     uint32_t lo[kRows], hi[kRows], mk[kRows];
 #pragma unroll
     for (int row = 0; row < kRows; ++row) {
-        row_window(a.cstart, sc, g, row % 3 - 1, row / 3 - 1, lo[row], hi[row]);
+        row_window(rcs, sc, g, row % 3 - 1, row / 3 - 1, lo[row], hi[row]);
         mk[row] = a.mask[(size_t)row * a.n + k];
     }
     {
-        const uint32_t b = k - lo[4];  // own bit, in the centre row
+        const uint32_t b = mask_bit(lo[4], hi[4], k);  // own bit, in the centre row
         if (b < (uint32_t)kMaskBits) mk[4] &= ~(1u << b);
     }
     ForceAcc f = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int row = 0; row < kRows; ++row) {
-        uint32_t m = mk[row];
-        const uint32_t l = lo[row];
-        while (m) {
-            const uint32_t j = l + (uint32_t)__builtin_ctz(m);
-            m &= m - 1u;
-            force_pair(sc, pi, vi, xi.x, load4(rpos, j << 4), load4(rvel, j << 4), load2(raux, j << 3), f);
-        }
-        if (hi[row] > l + (uint32_t)kMaskBits) {
-            for (uint32_t j = l + (uint32_t)kMaskBits; j < hi[row]; ++j) {
-                if (j == k) continue;
-                const float4 pj = a.pos_s[j];
-                if (dist2(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z) < sc.h2)
-                    force_pair(sc, pi, vi, xi.x, pj, a.vel_s[j], a.aux[j], f);
-            }
-        }
+        row_neighbors(
+            lo[row], hi[row], mk[row],
+            [&](uint32_t j) {
+                const float4 pj = load4(rpos, j << 4);
+                return j != k && dist2(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z) < sc.h2;
+            },
+            [&](uint32_t j) {
+                force_pair(sc, pi, vi, load4(rpos, j << 4), load4(rvel, j << 4), f);
+            });
     }
-    const float cvi = sc.cV * xi.y;
+    const float cvi = sc.cV * vi.w;
     const float a0 = (sc.cP * f.px + cvi * f.vx) + sc.g[0];
     const float a1 = (sc.cP * f.py + cvi * f.vy) + sc.g[1];
     const float a2 = (sc.cP * f.pz + cvi * f.vz) + sc.g[2];
@@ -148,41 +146,39 @@ __device__ __forceinline__ uint32_t force_one(const StepArgs& a, const SceneCons
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void density_kernel(StepArgs a) {
     const __amdgpu_buffer_rsrc_t rpos = buffer_of(a.pos_s, a.n);
-    uint32_t t0, t1;
-    tile_span(a.ntiles, t0, t1);
-    for (uint32_t t = t0; t < t1; ++t) {
+    const __amdgpu_buffer_rsrc_t rcs = buffer_of(a.cstart, a.ncells + 1u);
+    for_each_tile(a.ntiles, a.queue, [&](uint32_t t) {
         const uint2 ti = a.tiles[t];
         const SceneConst& sc = a.scenes[ti.x];
         const uint32_t k = ti.y + threadIdx.x;
-        if (k < sc.p_end) density_one(a, sc, k, rpos);
-    }
+        if (k < sc.p_end) density_one(a, sc, k, rpos, rcs);
+    });
 }
 
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void force_kernel(StepArgs a) {
     const __amdgpu_buffer_rsrc_t rpos = buffer_of(a.pos_s, a.n);
     const __amdgpu_buffer_rsrc_t rvel = buffer_of(a.vel_s, a.n);
-    const __amdgpu_buffer_rsrc_t raux = buffer_of(a.aux, a.n);
-    uint32_t t0, t1;
-    tile_span(a.ntiles, t0, t1);
-    for (uint32_t t = t0; t < t1; ++t) {
+    const __amdgpu_buffer_rsrc_t rcs = buffer_of(a.cstart, a.ncells + 1u);
+    uint32_t* queue = a.queue ? a.queue + 8 : nullptr;
+    for_each_tile(a.ntiles, queue, [&](uint32_t t) {
         const uint2 ti = a.tiles[t];
         const SceneConst& sc = a.scenes[ti.x];
         const uint32_t k = ti.y + threadIdx.x;
         const bool active = k < sc.p_end;
         uint32_t key = 0;
-        if (active) key = force_one(a, sc, k, rpos, rvel, raux);
+        if (active) key = force_one(a, sc, k, rpos, rvel, rcs);
         // Next step's key and its slot in the per-cell count (whole wave).
-        const uint32_t slot = count_slot(a.count, key, active);
+        const uint32_t slot = count_slot(a.count, a.count + a.ncells, key, active);
         if (active) {
             a.key0[k] = key;
             a.rank0[k] = slot;
         }
-    }
+    });
 }
 
 __global__ void hash_kernel(const float4* __restrict__ pos, uint32_t* __restrict__ key0,
-                            uint32_t* __restrict__ rank0, uint32_t* __restrict__ count,
+                            uint32_t* __restrict__ rank0, uint32_t* __restrict__ count, uint32_t ncells,
                             const SceneConst* __restrict__ scenes, const uint2* __restrict__ tiles,
                             uint32_t ntiles) {
     uint32_t t0, t1;
@@ -197,7 +193,7 @@ __global__ void hash_kernel(const float4* __restrict__ pos, uint32_t* __restrict
             const float4 p = pos[k];
             key = cell_key(sc, p.x, p.y, p.z);
         }
-        const uint32_t slot = count_slot(count, key, active);
+        const uint32_t slot = count_slot(count, count + ncells, key, active);
         if (active) {
             key0[k] = key;
             rank0[k] = slot;
@@ -209,6 +205,7 @@ __global__ void hash_kernel(const float4* __restrict__ pos, uint32_t* __restrict
 template <int FILL>
 __global__ void neighbors_kernel(StepArgs a, const uint32_t* __restrict__ offsets,
                                  uint32_t* __restrict__ out) {
+    const __amdgpu_buffer_rsrc_t rcs = buffer_of(a.cstart, a.ncells + 1u);
     uint32_t t0, t1;
     tile_span(a.ntiles, t0, t1);
     for (uint32_t t = t0; t < t1; ++t) {
@@ -222,22 +219,17 @@ __global__ void neighbors_kernel(StepArgs a, const uint32_t* __restrict__ offset
         const uint32_t o = FILL ? offsets[k] : 0u;
         for (int row = 0; row < kRows; ++row) {
             uint32_t lo, hi;
-            row_window(a.cstart, sc, g, row % 3 - 1, row / 3 - 1, lo, hi);
-            uint32_t m = a.mask[(size_t)row * a.n + k];
-            while (m) {
-                const uint32_t b = (uint32_t)__builtin_ctz(m);
-                m &= m - 1u;
-                if (FILL) out[o + c] = lo + b;
-                ++c;
-            }
-            if (hi > lo + (uint32_t)kMaskBits)
-                for (uint32_t j = lo + (uint32_t)kMaskBits; j < hi; ++j) {
+            row_window(rcs, sc, g, row % 3 - 1, row / 3 - 1, lo, hi);
+            row_neighbors(
+                lo, hi, a.mask[(size_t)row * a.n + k],
+                [&](uint32_t j) {
                     const float4 pj = a.pos_s[j];
-                    if (dist2(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z) < sc.h2) {
-                        if (FILL) out[o + c] = j;
-                        ++c;
-                    }
-                }
+                    return dist2(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z) < sc.h2;
+                },
+                [&](uint32_t j) {
+                    if (FILL) out[o + c] = j;
+                    ++c;
+                });
         }
         if (!FILL) out[k] = c;
     }
@@ -298,10 +290,11 @@ hipError_t launch_force(hipStream_t st, int block, uint32_t grid, const StepArgs
 }
 
 hipError_t launch_hash(hipStream_t st, int block, uint32_t grid, const float4* pos, uint32_t* key0,
-                       uint32_t* rank0, uint32_t* count, const SceneConst* scenes, const uint2* tiles,
-                       uint32_t ntiles) {
+                       uint32_t* rank0, uint32_t* count, uint32_t ncells, const SceneConst* scenes,
+                       const uint2* tiles, uint32_t ntiles) {
     if (grid == 0) return hipSuccess;
-    hipLaunchKernelGGL(hash_kernel, dim3(grid), dim3(block), 0, st, pos, key0, rank0, count, scenes, tiles, ntiles);
+    hipLaunchKernelGGL(hash_kernel, dim3(grid), dim3(block), 0, st, pos, key0, rank0, count, ncells, scenes, tiles,
+                       ntiles);
     return hipGetLastError();
 }
 

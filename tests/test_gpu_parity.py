@@ -186,7 +186,7 @@ def test_runs_are_deterministic_and_tuning_invariant(E):
     from simplefluid_amd import configs
     cfg = configs.dambreak_65k(jitter=1e-6, seed=4)
     res = []
-    for tune in ((256, 256, 256, 8), (256, 256, 256, 8), (64, 512, 128, 16), (512, 128, 64, 4)):
+    for tune in ((256, 256, 256, 8), (256, 256, 256, 8), (64, 512, 128, 16), (512, 128, 64, 4, 1)):
         e = make_engine(E, cfg["params"], cfg["pos"])
         e.set_tuning(*tune)
         e.advance_steps(30)
@@ -194,6 +194,23 @@ def test_runs_are_deterministic_and_tuning_invariant(E):
         e.close()
     for r in res[1:]:
         assert np.array_equal(res[0], r)
+
+
+def test_persistent_launch_matches_one_block_per_tile(E):
+    """Density and force as persistent blocks over per-XCD tile queues give
+    the same bits as one block per tile (1M particles: more tiles than
+    resident blocks, so the queues are in use)."""
+    from simplefluid_amd import configs
+    cfg = configs.double_dambreak_1m()
+    res = []
+    for tune in ((64, 64, 256, 8, 0), (64, 64, 256, 8, 1), (512, 256, 256, 8, 1)):
+        e = make_engine(E, cfg["params"], cfg["pos"])
+        e.set_tuning(*tune)
+        e.advance_steps(8)
+        res.append((e.get_particles(), e.get_density()))
+        e.close()
+    for r in res[1:]:
+        assert np.array_equal(res[0][0], r[0]) and np.array_equal(res[0][1], r[1])
 
 
 def test_sort_modes_agree(E):
