@@ -74,6 +74,11 @@ typedef struct sph_params {
     float box_max[3];
     float stop_time;                 /* driver only (Source/Simulator.cpp:44)     */
     int   num_threads;               /* accepted for parity, unused on the GPU    */
+    float cfl_number;                /* 0: every step is dt.  > 0: a step is
+                                        min(dt, cfl_number h / (c0 + max|v|)),
+                                        c0 = sqrt(7 B / rho0), max over the scene's
+                                        velocities at the start of the step
+                                        (single-scene handles only)              */
 } sph_params;
 
 typedef struct sph_handle sph_handle;
@@ -94,7 +99,8 @@ int  sph_get_params(const sph_handle* h, sph_params* out);
  * scenes set, Source/SceneManager.cpp:63).                                  */
 int  sph_set_particles(sph_handle* h, const float* xyz, const float* vel, uint32_t n);
 int  sph_make_ready(sph_handle* h);
-/* One step of dt; *dt_out receives dt (the value advanceFrame() returns).  */
+/* One step; *dt_out receives its length (the value advanceFrame() returns):
+ * dt, or the CFL step when cfl_number > 0 (then the call waits for the GPU). */
 int  sph_advance_frame(sph_handle* h, float* dt_out);
 /* k steps, replayed from a captured HIP graph when graphs are enabled.     */
 int  sph_advance_steps(sph_handle* h, uint32_t k);

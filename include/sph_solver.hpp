@@ -66,6 +66,7 @@ struct SPHParameters {
     float timestep;
     float boxMin[3];
     float boxMax[3];
+    float cflNumber;   // 0: fixed timestep; > 0: advanceFrame() takes the CFL step
 
     /* The GUI defaults (resolution 24 and the panel values). */
     SPHParameters() {
@@ -99,6 +100,7 @@ struct SPHParameters {
         p.dt = timestep;
         p.stop_time = stopTime;
         p.num_threads = numThreads;
+        p.cfl_number = cflNumber;
         return p;
     }
     void from_c(const sph_params& p) {
@@ -118,6 +120,7 @@ struct SPHParameters {
         timestep = p.dt;
         stopTime = p.stop_time;
         numThreads = p.num_threads;
+        cflNumber = p.cfl_number;
     }
 };
 

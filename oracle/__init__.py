@@ -35,6 +35,7 @@ class Params(C.Structure):
         ("box_min", C.c_float * 3),
         ("box_max", C.c_float * 3),
         ("x_subdivision", C.c_int),
+        ("cfl_number", C.c_float),
     ]
 
 
@@ -44,7 +45,7 @@ def make_params(**kw) -> Params:
              pressure_stiffness=50.0, viscosity=0.01, boundary_restitution=0.1,
              use_attractive_pressure=0, attractive_pressure_ratio=0.1,
              gravity=(0.0, -9.81, 0.0), dt=1e-4, box_min=(0.0, 0.0, 0.0),
-             box_max=(1.0, 1.0, 1.0), x_subdivision=1)
+             box_max=(1.0, 1.0, 1.0), x_subdivision=1, cfl_number=0.0)
     d.update(kw)
     for k, v in d.items():
         if k in ("gravity", "box_min", "box_max"):
@@ -96,6 +97,8 @@ def lib():
         L.orc_num_cells.restype = u32
         L.orc_num_cells.argtypes = [vp]
         L.orc_get_state.argtypes = [vp, vp, vp, vp]
+        L.orc_last_dt.restype = dbl
+        L.orc_last_dt.argtypes = [vp]
         L.orc_get_constants.argtypes = [vp, vp, vp]
         L.orc_get_step_debug.argtypes = [vp, vp, vp, vp, vp, vp, vp]
         L.orc_get_cells.argtypes = [vp, vp, vp]
@@ -144,6 +147,10 @@ class Oracle:
 
     def step(self, k: int = 1) -> float:
         return self._L.orc_step(self._h, int(k))
+
+    def last_dt(self) -> float:
+        """Step length of the last step (dt, or the CFL step)."""
+        return self._L.orc_last_dt(self._h)
 
     @property
     def n(self) -> int:

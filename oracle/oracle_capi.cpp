@@ -100,6 +100,12 @@ uint32_t orc_num_cells(void* hp) {
 }
 
 // Current state in state order (the sorted order of the last step).
+// Step length of the last step (dt, or the CFL step).
+double orc_last_dt(void* hp) {
+    Handle* h = (Handle*)hp;
+    return h->fp64 ? (double)h->d.c.dt : (double)h->f.c.dt;
+}
+
 void orc_get_state(void* hp, double* pos3, double* vel3, uint32_t* ids) {
     Handle* h = (Handle*)hp;
     if (h->fp64) get_state_t(h->d, pos3, vel3, ids); else get_state_t(h->f, pos3, vel3, ids);

@@ -56,11 +56,13 @@ struct Params {
     float box_min[3];
     float box_max[3];
     int   x_subdivision;   // x cells per y/z cell (1, 2 or 4)
+    float cfl_number;      // 0: fixed dt; > 0: CFL step capped at dt
 };
 
 template <class T>
 struct Consts {
     T h, h2, inv_cs, inv_csx, mass, cD, cP, cV, inv_rho0, B, e, attr, dt, r;
+    T cfl, c0, dt_max;  // CFL number, EOS sound speed sqrt(7 B / rho0), dt cap
     int xsub;
     T g[3], bmin[3], lo[3], hi[3];
     int G[3];
@@ -97,6 +99,9 @@ inline Consts<T> derive(const Params& p) {
     c.attr      = (T)p.attractive_pressure_ratio;
     c.attractive = p.use_attractive_pressure != 0;
     c.dt        = (T)p.dt;
+    c.dt_max    = (T)p.dt;
+    c.cfl       = (T)p.cfl_number;
+    c.c0        = (T)std::sqrt(7.0 * (double)p.pressure_stiffness / (double)p.rest_density);
     c.ncells    = 1;
     for (int k = 0; k < 3; ++k) {
         c.g[k]    = (T)p.gravity[k];
@@ -247,9 +252,24 @@ struct Sim {
         cz = cell_coord(z_, c.bmin[2], c.inv_cs, c.G[2]);
     }
 
+    // Adaptive step (cfl > 0): dt = min(dt_max, (cfl h) / (c0 + max|v|)) from
+    // the velocities the step starts from, in this op order.  Engine-defined
+    // (the reference's advanceFrame returns a variable step,
+    // Source/Simulator.cpp:46-51, by a rule not in the repository).
+    void cfl_step() {
+        if (!(c.cfl > T(0))) return;
+        T vmax2 = T(0);
+        for (uint32_t i = 0; i < n; ++i) {
+            const T v2 = dist2(vx[i], vy[i], vz[i]);
+            if (v2 > vmax2) vmax2 = v2;
+        }
+        c.dt = std::fmin(c.dt_max, (c.cfl * c.h) / (c.c0 + std::sqrt(vmax2)));
+    }
+
     void step() {
         const int nt = std::max(1, nthreads);
         (void)nt;
+        cfl_step();
         key.resize(n);
 #pragma omp parallel for num_threads(nt) schedule(static)
         for (long long i = 0; i < (long long)n; ++i) key[i] = cell_key(c, x[i], y[i], z[i]);

@@ -48,9 +48,12 @@ struct SceneConst {
     float xsubf;       // x cells per cs (1, 2 or 4)
     float hc2;         // (h * inv_cs)^2: the support radius squared in y/z cell units
     float wmargin;     // x-window safety margin in x cells
-    float pad;
+    float cfl;         // CFL number (0: fixed dt)
+    float c0;          // sqrt(7 B / rho0), the EOS sound speed
+    float dt_max;      // the parameter dt: the step, or its cap under CFL
+    uint32_t pad[2];
 };
-static_assert(sizeof(SceneConst) == 144, "SceneConst layout");
+static_assert(sizeof(SceneConst) == 160, "SceneConst layout");
 
 constexpr int kRows = 9;          // 3x3 (dz, dy) rows of three x-adjacent cells
 constexpr int kMaskBits = 32;     // candidates per row tracked in the mask
@@ -255,6 +258,17 @@ __device__ __forceinline__ uint32_t count_slot(uint32_t* __restrict__ count, uin
     const uint32_t slot = run_add(count, key, active);
     (void)run_add(tsum, key >> kScanTileShift, active);
     return slot;
+}
+
+// Adds the wave's largest |v|^2 to arr[idx] (float bits; atomic max on the
+// bit patterns orders non-negative floats).  NaN counts as 0.  All lanes of
+// the wave must call it, with idx uniform.
+__device__ __forceinline__ void wave_max_v2(uint32_t* __restrict__ arr, uint32_t idx, float vx, float vy, float vz,
+                                            bool active) {
+    float v2 = active ? fmaxf(fmaf(vz, vz, fmaf(vy, vy, vx * vx)), 0.0f) : 0.0f;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v2 = fmaxf(v2, __shfl_xor(v2, o, 64));
+    if ((threadIdx.x & 63u) == 0u) atomicMax(arr + idx, __float_as_uint(v2));
 }
 
 // Tait EOS, exponent 7 by the chain x^2, x^4, (x^4 x^2) x; p < 0 is clamped

@@ -75,6 +75,10 @@ int derive_scene(const sph_params& p, SceneConst& sc, std::string& err, int xsub
     sc.attractive = p.use_attractive_pressure ? 1 : 0;
     sc.e = p.boundary_restitution;
     sc.dt = p.dt;
+    sc.dt_max = p.dt;
+    if (!(p.cfl_number >= 0.f) || !std::isfinite(p.cfl_number)) { err = "cfl_number must be >= 0"; return SPH_E_INVALID; }
+    sc.cfl = p.cfl_number;
+    sc.c0 = (float)std::sqrt(7.0 * (double)p.pressure_stiffness / (double)p.rest_density);
     sc.mu = p.viscosity;
     uint64_t cells = 1;
     for (int k = 0; k < 3; ++k) {
@@ -196,12 +200,15 @@ static int ensure_buffers(sph_handle* h) {
     chk(h->tickets.ensure(kMaxPasses));
     chk(h->err.ensure(1));
     chk(h->queue.ensure(kQueueWords));
+    chk(h->vmax2.ensure(h->sc_h.size()));
+    chk(h->dt_dev.ensure(1));
     chk(h->staging.ensure(3 * n + 1));
     chk(h->sc_d.ensure(h->sc_h.size()));
     if (e != hipSuccess) return hip_fail(h, e, "device allocation");
     chk(launch_fill_u32(h->stream, h->hist.p, 0u, kMaxPasses * 256));
     chk(launch_fill_u32(h->stream, h->err.p, 0u, 1));
     chk(launch_fill_u32(h->stream, h->queue.p, 0u, kQueueWords));
+    chk(launch_fill_u32(h->stream, h->vmax2.p, 0u, h->sc_h.size()));
     if (fresh_count) chk(launch_fill_u32(h->stream, h->count.p, 0u, count_words));
     chk(hipMemcpyAsync(h->sc_d.p, h->sc_h.data(), h->sc_h.size() * sizeof(SceneConst), hipMemcpyHostToDevice, h->stream));
     if (e != hipSuccess) return hip_fail(h, e, "buffer init");
@@ -224,11 +231,12 @@ int enqueue_step(sph_handle* h, StepBufs& w, bool record_prof) {
     if (!h->keys_valid) {
         // New state: counts start from zero, then every particle is hashed.
         hipError_t e = launch_fill_u32(st, h->count.p, 0u, (size_t)h->ncells + 2 * (size_t)scan_tiles(h->ncells));
+        if (e == hipSuccess && h->cfl) e = launch_fill_u32(st, h->vmax2.p, 0u, h->sc_h.size());
         uint32_t nt = 0;
         const uint2* tl = tiles_for(h, 256, nt);
         if (e == hipSuccess)
-            e = launch_hash(st, 256, nt, h->pos.p, h->key0.p, h->rank0.p, h->count.p, h->ncells, h->sc_d.p, tl,
-                            nt);
+            e = launch_hash(st, 256, nt, h->pos.p, h->vel.p, h->key0.p, h->rank0.p, h->count.p, h->ncells,
+                            h->cfl ? h->vmax2.p : nullptr, h->sc_d.p, tl, nt);
         if (e != hipSuccess) return hip_fail(h, e, "hash launch");
         h->keys_valid = true;
     }
@@ -272,8 +280,14 @@ int enqueue_step(sph_handle* h, StepBufs& w, bool record_prof) {
         if (e != hipSuccess) return hip_fail(h, e, "density launch");
     }
     if (record_prof) prof_record(h, 4);
+    if (h->cfl) {
+        // Step length from the largest speed of the state this step starts from.
+        e = launch_cfl(st, (uint32_t)h->sc_h.size(), h->sc_d.p, h->vmax2.p, h->dt_dev.p);
+        if (e != hipSuccess) return hip_fail(h, e, "cfl launch");
+    }
     {
         StepArgs a = step_args(h, w, h->tune.force_block);
+        a.vmax2 = h->cfl ? h->vmax2.p : nullptr;
         const uint32_t grid = grid_for(h, h->tune.force_block, a.ntiles, &a.queue);
         e = launch_force(st, h->tune.force_block, grid, a);
         if (e != hipSuccess) return hip_fail(h, e, "force launch");
@@ -343,6 +357,8 @@ static int configure(sph_handle* h, const std::vector<sph_params>& ps, const std
         const int rc = derive_scene(ps[s], sc[s], err, xsub);
         if (rc != SPH_OK) return fail(h, rc, ps.size() > 1 ? "scene " + std::to_string(s) + ": " + err : err);
         if (ps[s].dt != ps[0].dt) return fail(h, SPH_E_INVALID, "all scenes of a batch must share dt");
+        if (ps.size() > 1 && ps[s].cfl_number > 0.f)
+            return fail(h, SPH_E_INVALID, "the CFL step is for single-scene handles (a batch shares dt)");
         sc[s].cell_offset = (uint32_t)cell_off;
         cell_off += (uint64_t)sc[s].G[0] * sc[s].G[1] * sc[s].G[2];
         sc[s].p_begin = (uint32_t)p_off;
@@ -360,6 +376,7 @@ static int configure(sph_handle* h, const std::vector<sph_params>& ps, const std
     h->ncells = (uint32_t)cell_off;
     h->passes = sort_key_passes(h->ncells);
     h->xsub = xsub;
+    h->cfl = ps.size() == 1 && ps[0].cfl_number > 0.f;
     h->tiles.clear();
     h->ntiles.clear();
     invalidate_graph(h);
@@ -550,7 +567,14 @@ int sph_advance_frame(sph_handle* h, float* dt_out) {
     if (!h) return SPH_E_INVALID;
     int rc = sph_advance_steps(h, 1);
     if (rc != SPH_OK) return rc;
-    if (dt_out) *dt_out = h->params[0].dt;
+    float dt = h->params[0].dt;
+    if (h->cfl) {
+        const hipError_t e = hipMemcpyAsync(&dt, h->dt_dev.p, sizeof(float), hipMemcpyDeviceToHost, h->stream);
+        if (e != hipSuccess) return hip_fail(h, e, "step length read");
+        rc = sync(h);
+        if (rc != SPH_OK) return rc;
+    }
+    if (dt_out) *dt_out = dt;
     return SPH_OK;
 }
 

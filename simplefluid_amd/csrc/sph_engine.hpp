@@ -103,6 +103,9 @@ struct sph_handle {
     sph::DevBuf<uint32_t> id, key0, rank0, count;
     sph::StepBufs work;
     sph::DevBuf<uint32_t> hist, status, tickets, err, queue;
+    bool cfl = false;                      // adaptive step (single scene)
+    sph::DevBuf<uint32_t> vmax2;           // per-scene max |v|^2 bits for the next step
+    sph::DevBuf<float> dt_dev;             // step length of the last step (scene 0)
     sph::DevBuf<sph::SceneConst> sc_d;
     std::map<int, sph::DevBuf<uint2>> tiles;
     std::map<int, uint32_t> ntiles;

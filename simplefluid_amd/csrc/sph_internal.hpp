@@ -74,15 +74,19 @@ struct StepArgs {
     uint32_t n;
     uint32_t ncells;
     uint32_t* queue;         // persistent launch: 8 tile counters (zeroed by the scan), else null
+    uint32_t* vmax2;         // CFL: per-scene max |v|^2 (float bits) of the new state, else null
 };
 
 // Tile counters of the persistent density and force launches (8 each),
 // zeroed by the cell-start scan of every step.
 constexpr int kQueueWords = 16;
 
-hipError_t launch_hash(hipStream_t st, int block, uint32_t grid, const float4* pos, uint32_t* key0,
-                       uint32_t* rank0, uint32_t* count, uint32_t ncells, const SceneConst* scenes,
-                       const uint2* tiles, uint32_t ntiles);
+hipError_t launch_hash(hipStream_t st, int block, uint32_t grid, const float4* pos, const float4* vel,
+                       uint32_t* key0, uint32_t* rank0, uint32_t* count, uint32_t ncells, uint32_t* vmax2,
+                       const SceneConst* scenes, const uint2* tiles, uint32_t ntiles);
+// CFL step of every scene from its max |v|^2 (scenes with cfl = 0 keep dt);
+// clears vmax2 and stores scene 0's step in *dt_out.
+hipError_t launch_cfl(hipStream_t st, uint32_t nscenes, SceneConst* scenes, uint32_t* vmax2, float* dt_out);
 hipError_t launch_density(hipStream_t st, int block, uint32_t grid, const StepArgs& a);
 hipError_t launch_force(hipStream_t st, int block, uint32_t grid, const StepArgs& a);
 hipError_t launch_fill_u32(hipStream_t st, uint32_t* p, uint32_t value, size_t count);

@@ -43,6 +43,7 @@ int sph_rollout_forward(sph_handle* h, uint32_t steps, uint32_t every) {
     if (!h) return SPH_E_INVALID;
     if (!h->ready) return fail(h, SPH_E_NOT_READY, "call sph_make_ready first");
     if (steps == 0 || every == 0) return fail(h, SPH_E_INVALID, "steps and checkpoint interval must be > 0");
+    if (h->cfl) return fail(h, SPH_E_INVALID, "the differentiable rollout needs a fixed dt (cfl_number = 0)");
     if (!h->rollout) h->rollout.reset(new (std::nothrow) RolloutState());
     if (!h->rollout) return fail(h, SPH_E_OOM, "rollout state");
     RolloutState& R = *h->rollout;

@@ -170,6 +170,11 @@ __global__ __launch_bounds__(BLOCK) void force_kernel(StepArgs a) {
         if (active) key = force_one(a, sc, k, rpos, rvel, rcs);
         // Next step's key and its slot in the per-cell count (whole wave).
         const uint32_t slot = count_slot(a.count, a.count + a.ncells, key, active);
+        if (a.vmax2) {
+            float4 vn = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (active) vn = a.vel_o[k];
+            wave_max_v2(a.vmax2, ti.x, vn.x, vn.y, vn.z, active);
+        }
         if (active) {
             a.key0[k] = key;
             a.rank0[k] = slot;
@@ -177,10 +182,10 @@ __global__ __launch_bounds__(BLOCK) void force_kernel(StepArgs a) {
     });
 }
 
-__global__ void hash_kernel(const float4* __restrict__ pos, uint32_t* __restrict__ key0,
-                            uint32_t* __restrict__ rank0, uint32_t* __restrict__ count, uint32_t ncells,
-                            const SceneConst* __restrict__ scenes, const uint2* __restrict__ tiles,
-                            uint32_t ntiles) {
+__global__ void hash_kernel(const float4* __restrict__ pos, const float4* __restrict__ vel,
+                            uint32_t* __restrict__ key0, uint32_t* __restrict__ rank0, uint32_t* __restrict__ count,
+                            uint32_t ncells, uint32_t* __restrict__ vmax2, const SceneConst* __restrict__ scenes,
+                            const uint2* __restrict__ tiles, uint32_t ntiles) {
     uint32_t t0, t1;
     tile_span(ntiles, t0, t1);
     for (uint32_t t = t0; t < t1; ++t) {
@@ -197,6 +202,11 @@ __global__ void hash_kernel(const float4* __restrict__ pos, uint32_t* __restrict
         if (active) {
             key0[k] = key;
             rank0[k] = slot;
+        }
+        if (vmax2) {
+            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (active) v = vel[k];
+            wave_max_v2(vmax2, ti.x, v.x, v.y, v.z, active);
         }
     }
 }
@@ -233,6 +243,20 @@ __global__ void neighbors_kernel(StepArgs a, const uint32_t* __restrict__ offset
         }
         if (!FILL) out[k] = c;
     }
+}
+
+// dt = min(dt_max, (cfl h) / (c0 + |v|max)) per scene, in this op order.
+__global__ void cfl_kernel(uint32_t nscenes, SceneConst* __restrict__ scenes, uint32_t* __restrict__ vmax2,
+                           float* __restrict__ dt_out) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nscenes) return;
+    SceneConst& sc = scenes[s];
+    if (sc.cfl > 0.0f) {
+        const float vmax = sqrtf(__uint_as_float(vmax2[s]));
+        sc.dt = fminf(sc.dt_max, (sc.cfl * sc.h) / (sc.c0 + vmax));
+    }
+    vmax2[s] = 0u;
+    if (s == 0) *dt_out = sc.dt;
 }
 
 __global__ void fill_u32_kernel(uint32_t* __restrict__ p, uint32_t v, size_t count) {
@@ -289,12 +313,12 @@ hipError_t launch_force(hipStream_t st, int block, uint32_t grid, const StepArgs
     return hipGetLastError();
 }
 
-hipError_t launch_hash(hipStream_t st, int block, uint32_t grid, const float4* pos, uint32_t* key0,
-                       uint32_t* rank0, uint32_t* count, uint32_t ncells, const SceneConst* scenes,
-                       const uint2* tiles, uint32_t ntiles) {
+hipError_t launch_hash(hipStream_t st, int block, uint32_t grid, const float4* pos, const float4* vel,
+                       uint32_t* key0, uint32_t* rank0, uint32_t* count, uint32_t ncells, uint32_t* vmax2,
+                       const SceneConst* scenes, const uint2* tiles, uint32_t ntiles) {
     if (grid == 0) return hipSuccess;
-    hipLaunchKernelGGL(hash_kernel, dim3(grid), dim3(block), 0, st, pos, key0, rank0, count, ncells, scenes, tiles,
-                       ntiles);
+    hipLaunchKernelGGL(hash_kernel, dim3(grid), dim3(block), 0, st, pos, vel, key0, rank0, count, ncells, vmax2,
+                       scenes, tiles, ntiles);
     return hipGetLastError();
 }
 
@@ -305,6 +329,12 @@ hipError_t launch_neighbors(hipStream_t st, int block, uint32_t grid, const Step
         hipLaunchKernelGGL(neighbors_kernel<1>, dim3(grid), dim3(block), 0, st, a, offsets, out);
     else
         hipLaunchKernelGGL(neighbors_kernel<0>, dim3(grid), dim3(block), 0, st, a, offsets, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_cfl(hipStream_t st, uint32_t nscenes, SceneConst* scenes, uint32_t* vmax2, float* dt_out) {
+    if (nscenes == 0) return hipSuccess;
+    hipLaunchKernelGGL(cfl_kernel, dim3((nscenes + 255) / 256), dim3(256), 0, st, nscenes, scenes, vmax2, dt_out);
     return hipGetLastError();
 }
 

@@ -81,6 +81,7 @@ class sph_params(C.Structure):
         ("box_max", C.c_float * 3),
         ("stop_time", C.c_float),
         ("num_threads", C.c_int),
+        ("cfl_number", C.c_float),
     ]
 
 
@@ -197,7 +198,7 @@ def make_params(**kw) -> sph_params:
                 pressure_stiffness=50.0, viscosity=0.01, boundary_restitution=0.1,
                 use_attractive_pressure=0, attractive_pressure_ratio=0.1,
                 gravity=(0.0, -9.81, 0.0), dt=1e-4, box_min=(0.0, 0.0, 0.0),
-                box_max=(1.0, 1.0, 1.0), stop_time=5.0, num_threads=0)
+                box_max=(1.0, 1.0, 1.0), stop_time=5.0, num_threads=0, cfl_number=0.0)
     base.update(kw)
     return set_fields(p, **base)
 

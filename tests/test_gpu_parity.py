@@ -244,6 +244,37 @@ def test_cell_subdivision_parity(E, xsub):
     step_and_compare(E, e, cfg["params"], f"xsub {xsub}")
 
 
+def test_cfl_step_parity(E):
+    """Adaptive step: the engine's step lengths follow the oracle's CFL rule
+    (identical first step from rest, then within 1e-4 as the trajectories
+    differ by round-off), a step from an identical state matches the oracle
+    in every intermediate, and a frame sums steps to >= 1/30 s."""
+    import oracle
+    from simplefluid_amd import configs
+    cfg = configs.dambreak_65k(jitter=1e-6, seed=21)
+    p = E.make_params(pressure_stiffness=5000.0, viscosity=0.01, dt=5e-3, cfl_number=0.3)
+    e = make_engine(E, p, cfg["pos"])
+    o = oracle.Oracle(to_oracle(p, e.debug_constants()["xsub"]), threads=16)
+    o.set_state(cfg["pos"].astype(np.float64))
+    dts = []
+    for s in range(20):
+        dts.append(e.advance_frame())
+        o.step()
+        if s == 0:
+            assert dts[0] == np.float32(o.last_dt())
+        assert abs(dts[-1] - o.last_dt()) <= 1e-4 * o.last_dt(), (s, dts[-1], o.last_dt())
+    say(f"[cfl] steps {dts[0]:.3e} .. {dts[-1]:.3e}")
+    assert dts[-1] < dts[0] <= 5e-3          # the fluid speeds up, steps shrink
+    step_and_compare(E, e, p, "cfl step")
+    t, n = e.run_frame()
+    assert t >= np.float32(0.0333333333) and n >= 1
+    with pytest.raises(E.SPHError):
+        e.rollout_forward(10, 5)            # the adjoint needs a fixed dt
+    e.close()
+    with pytest.raises(E.SPHError):
+        E.BatchEngine([p, p], [10, 10])     # a batch shares one dt
+
+
 def test_get_particles_original_order(E):
     from simplefluid_amd import configs
     cfg = configs.dambreak_65k()

@@ -272,6 +272,35 @@ def test_wall_clamp_and_restitution(oracle_lib):
     assert v[0, 1] == pytest.approx(-f(0.25) * vy, rel=1e-12)
 
 
+def test_cfl_step_closed_form(oracle_lib):
+    """CFL step = min(dt, cfl h / (c0 + max|v|)), c0 = sqrt(7 B / rho0), from
+    the velocities a step starts from; float32 arithmetic in that order.  A
+    planted error (c0 without the factor 7) would change the step."""
+    import oracle
+    f32 = np.float32
+    rng = np.random.default_rng(5)
+    pos = lattice((6, 6, 6), jitter=1e-4, seed=5)
+    vel = rng.normal(0.0, 0.7, size=pos.shape)
+    for cfl, dt_max, B in ((0.3, 5e-3, 5000.0), (0.3, 1e-4, 5000.0), (0.0, 1e-4, 50.0)):
+        p = oracle.make_params(cfl_number=cfl, dt=dt_max, pressure_stiffness=B)
+        o = oracle.Oracle(p, fp64=False)
+        o.set_state(pos, vel)
+        o.step()
+        v32 = vel.astype(np.float32)
+        v2 = (v32[:, 2] * v32[:, 2] + (v32[:, 1] * v32[:, 1] + v32[:, 0] * v32[:, 0])).max()
+        # fma chain vs mul+add differ by one rounding at most: compare closed
+        # form in float64 to float32 resolution.
+        c0 = f32(np.sqrt(7.0 * f32(B) / 1000.0))
+        want = min(f32(dt_max), (f32(cfl) * f32(0.02)) / (c0 + f32(np.sqrt(v2)))) if cfl > 0 else f32(dt_max)
+        assert o.last_dt() == pytest.approx(float(want), rel=3e-7), (cfl, dt_max, B)
+    # Zero velocity: exactly cfl h / c0 when below the cap.
+    p = oracle.make_params(cfl_number=0.25, dt=1.0, pressure_stiffness=7000.0)
+    o = oracle.Oracle(p, fp64=True)
+    o.set_state(pos)
+    o.step()
+    assert o.last_dt() == pytest.approx(0.25 * float(f32(0.02)) / np.sqrt(7.0 * 7000.0 / 1000.0), rel=1e-7)
+
+
 def test_float_vs_double_step_agree(oracle_lib):
     import oracle
     pos = lattice((10, 20, 10), jitter=1e-4, seed=9)
