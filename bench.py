@@ -9,10 +9,13 @@ Prints ONE JSON line.  `value` is particle-steps/s over K graph-replayed
 steps with the state resident in HBM.  `roofline` is the dominant kernel's
 algorithmic bytes per launch over its mean duration, measured with HIP events
 on the engine's stream in a second run of K steps; `cpu_baseline` times the
-oracle's OpenMP C++ step on this host over a bounded sample.  Extras:
-`batched` (256-scene ensemble, one launch per pass, vs the same scenes
-launched one engine at a time) and `adjoint` (rollout reverse pass vs
-forward wall-clock, 16 x 32,768 particles, 200 steps, checkpoint every 20).
+oracle's OpenMP C++ step on this host over a bounded sample (all host
+threads, and one thread).  Extras: `batched` (256-scene ensemble, one launch
+per pass, vs the same scenes launched one engine at a time), `adjoint`
+(rollout reverse pass vs forward wall-clock, 16 x 32,768 particles, 200
+steps, checkpoint every 20), `dambreak_65k` (the 65,536-particle dam break,
+1000 steps) and `sphere_4m` (the 4.19M-particle sphere drop over a resting
+layer, r scaled to 0.0015).
 """
 from __future__ import annotations
 
@@ -62,16 +65,23 @@ def plain(args):
     log("tuning", e.get_tuning())
     e.advance_steps(args.warmup)
     e.synchronize()
+    start = e.debug_get_state()
     t = time_steps(e, args.steps)
     value = n * args.steps / t
     log(f"plain: {n} particles, {args.steps} steps in {t:.4f}s -> {value:.4e} particle-steps/s")
-    # Per-pass durations: K more steps with HIP events between passes.
+    # Per-pass durations over the same K steps again (the state is restored to
+    # the start of the timed region), with HIP events between the passes on
+    # the engine's stream.
+    e.debug_set_state(*start)
     e.profile_enable(True)
     e.advance_steps(args.steps)
     ms, nst = e.profile_read()
     e.profile_enable(False)
     per = {k: v / max(nst, 1) for k, v in ms.items()}
     log("ms per pass", {k: round(v, 5) for k, v in per.items()})
+    t0 = time.perf_counter()
+    e.get_particles()
+    download_ms = (time.perf_counter() - t0) * 1e3
     s, en = e.debug_cells()
     occ = int((en > 0).sum())
     b_step = sum(v for k, v in B_ALG.items() if k != "hash") + 20.0 * 0 + 8.0 * occ / n
@@ -88,7 +98,8 @@ def plain(args):
              "step_achieved_GBs": round(step_roof, 1),
              "step_frac": round(step_roof / HBM_PEAK_GBS, 4),
              "pass_ms": {k: round(v, 5) for k, v in per.items()},
-             "occupied_cells": occ, "tuning": e.get_tuning()}
+             "occupied_cells": occ, "tuning": e.get_tuning(),
+             "download_ms": round(download_ms, 3)}
     e.close()
     return value, step_ms, roof, extra, cfg
 
@@ -108,18 +119,25 @@ def load_traffic(kernel):
 def cpu_baseline(cfg, budget_s):
     import oracle
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
-    o = oracle.Oracle(oracle.from_params(cfg["params"], 4), threads=threads)
-    o.set_state(cfg["pos"].astype(np.float64))
-    o.step(1)  # warm-up
-    steps, spent = 0, 0.0
-    while spent < budget_s and steps < 50:
-        spent += o.step(1)
-        steps += 1
     n = cfg["pos"].shape[0]
+
+    def run(nthreads, budget, max_steps):
+        o = oracle.Oracle(oracle.from_params(cfg["params"], 4), threads=nthreads)
+        o.set_state(cfg["pos"].astype(np.float64))
+        o.step(1)  # warm-up
+        steps, spent = 0, 0.0
+        while spent < budget and steps < max_steps:
+            spent += o.step(1)
+            steps += 1
+        return steps, spent
+
+    steps, spent = run(threads, budget_s, 50)
+    s1, t1 = run(1, budget_s / 3.0, 5)
     return {"value": round(n * steps / spent, 1), "unit": "particle-steps/s", "cores": threads,
             "kind": "port",
             "sample": f"oracle C++ FP32 step (OpenMP), {n} particles of the same workload, {steps} steps "
-                      f"after 1 warm-up, {spent:.1f}s"}
+                      f"after 1 warm-up, {spent:.1f}s",
+            "single_thread": {"value": round(n * s1 / t1, 1), "steps": s1, "seconds": round(t1, 2)}}
 
 
 def batched(args):
@@ -161,6 +179,42 @@ def batched(args):
     return {"scenes": 256, "particles": N, "steps": k, "value": round(vb, 1),
             "per_scene_value": round(vs, 1), "speedup": round(vb / vs, 3),
             "unit": "particle-steps/s"}
+
+
+def small_and_large(args):
+    """The 65k dam break over its full 1000 steps, and the 4M sphere drop."""
+    from simplefluid_amd import configs, engine
+    out = {}
+    cfg = configs.dambreak_65k()
+    e = engine.Engine(cfg["params"])
+    e.set_particles(cfg["pos"])
+    e.make_ready()
+    e.advance_steps(5)
+    t = time_steps(e, cfg["steps"])
+    n = cfg["pos"].shape[0]
+    out["dambreak_65k"] = {"particles": n, "steps": cfg["steps"], "value": round(n * cfg["steps"] / t, 1),
+                           "ms_per_step": round(t / cfg["steps"] * 1e3, 5), "unit": "particle-steps/s"}
+    e.close()
+    log("dambreak_65k", out["dambreak_65k"])
+    cfg = configs.sphere_drop_4m()
+    n = cfg["pos"].shape[0]
+    e = engine.Engine(cfg["params"])
+    e.set_particles(cfg["pos"])
+    e.make_ready()
+    e.advance_steps(5)
+    k = min(args.sphere_steps, cfg["steps"])
+    t = time_steps(e, k)
+    e.profile_enable(True)
+    e.advance_steps(20)
+    ms, nst = e.profile_read()
+    e.profile_enable(False)
+    out["sphere_4m"] = {"particles": n, "steps": k, "value": round(n * k / t, 1),
+                        "ms_per_step": round(t / k * 1e3, 5), "unit": "particle-steps/s",
+                        "pass_ms": {kk: round(v / max(nst, 1), 5) for kk, v in ms.items()},
+                        "x_subdivision": int(e.debug_constants()["xsub"])}
+    e.close()
+    log("sphere_4m", out["sphere_4m"])
+    return out
 
 
 def adjoint(args):
@@ -211,6 +265,7 @@ def main():
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--batch-steps", type=int, default=100)
+    ap.add_argument("--sphere-steps", type=int, default=100)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
@@ -248,6 +303,7 @@ def main():
     if not args.no_extras:
         out["batched"] = batched(args)
         out["adjoint"] = adjoint(args)
+        out.update(small_and_large(args))
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_seconds)
     print(json.dumps(out), flush=True)

@@ -113,6 +113,11 @@ int  sph_get_particles(sph_handle* h, float* xyz_out);
 int  sph_get_velocity(sph_handle* h, float* vel_out);
 int  sph_get_density(sph_handle* h, float* rho_out);   /* density of the last step */
 int  sph_synchronize(sph_handle* h);
+/* State dump of one frame (no reference counterpart: the reference exports
+ * only screenshots, Source/MainWindow.cpp:108-113): a NumPy .npy file of
+ * float32, shape (n, 6) = x, y, z, vx, vy, vz per particle in original
+ * order.  Waits for the GPU.                                                 */
+int  sph_export_frame(sph_handle* h, const char* path);
 const char* sph_last_error(const sph_handle* h);
 /* Graph capture of the step on (default) or off.                           */
 int  sph_set_use_graph(sph_handle* h, int on);

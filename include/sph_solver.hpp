@@ -189,6 +189,8 @@ public:
         refresh();
         return velocities_;
     }
+    /* State dump of the current frame (.npy, float32 (n, 6): x, v in original order). */
+    void exportFrame(const std::string& path) { check(sph_export_frame(h_, path.c_str())); }
     std::shared_ptr<SPHParameters> getParameters() const { return params_; }
     sph_handle* handle() const { return h_; }
 

@@ -4,11 +4,13 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 
 using namespace sph;
 
 sph_handle::~sph_handle() {
+    if (host_pinned) (void)hipHostFree(host_pinned);
     if (gexec) (void)hipGraphExecDestroy(gexec);
     if (graph) (void)hipGraphDestroy(graph);
     for (auto e : prof_events) (void)hipEventDestroy(e);
@@ -618,14 +620,54 @@ int sph_run_frame(sph_handle* h, float* frame_time_out, uint32_t* steps_out) {
 
 uint32_t sph_num_particles(const sph_handle* h) { return h ? h->n : 0u; }
 
+// .npy (format 1.0): magic, version, little-endian header length, then a
+// Python dict literal padded with spaces to a 64-byte boundary ending in '\n'.
+static bool write_npy_f32(const char* path, const float* data, uint32_t rows, uint32_t cols) {
+    std::string dict = "{'descr': '<f4', 'fortran_order': False, 'shape': (" + std::to_string(rows) + ", " +
+                       std::to_string(cols) + "), }";
+    const size_t pre = 10;
+    size_t total = pre + dict.size() + 1;
+    const size_t pad = (64 - total % 64) % 64;
+    dict.append(pad, ' ');
+    dict.push_back('\n');
+    const uint16_t hlen = (uint16_t)dict.size();
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return false;
+    const unsigned char head[8] = {0x93, 'N', 'U', 'M', 'P', 'Y', 1, 0};
+    bool ok = std::fwrite(head, 1, 8, f) == 8;
+    const unsigned char hl[2] = {(unsigned char)(hlen & 0xff), (unsigned char)(hlen >> 8)};
+    ok = ok && std::fwrite(hl, 1, 2, f) == 2;
+    ok = ok && std::fwrite(dict.data(), 1, dict.size(), f) == dict.size();
+    ok = ok && std::fwrite(data, sizeof(float), (size_t)rows * cols, f) == (size_t)rows * cols;
+    ok = (std::fclose(f) == 0) && ok;
+    return ok;
+}
+
 static int fetch3(sph_handle* h, const float4* src, float* out) {
     if (!h || !out) return SPH_E_INVALID;
     if (!h->has_particles) return fail(h, SPH_E_NOT_READY, "no particles set");
     if (h->n == 0) return SPH_OK;
+    const size_t cnt = 3 * (size_t)h->n;
+    // Download through a page-locked buffer of the handle (a DMA copy at full
+    // link rate), then one host copy into the caller's array.
+    if (h->host_pinned_n < cnt) {
+        if (h->host_pinned) (void)hipHostFree(h->host_pinned);
+        h->host_pinned = nullptr;
+        h->host_pinned_n = 0;
+        const hipError_t ae = hipHostMalloc((void**)&h->host_pinned, cnt * sizeof(float), hipHostMallocDefault);
+        if (ae != hipSuccess) {
+            h->host_pinned = nullptr;
+            return hip_fail(h, ae, "pinned host buffer");
+        }
+        h->host_pinned_n = cnt;
+    }
     hipError_t e = launch_unpermute3(h->stream, h->n, src, h->id.p, h->staging.p);
-    if (e == hipSuccess) e = hipMemcpyAsync(out, h->staging.p, 3 * (size_t)h->n * sizeof(float), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(h->host_pinned, h->staging.p, cnt * sizeof(float), hipMemcpyDeviceToHost, h->stream);
     if (e != hipSuccess) return hip_fail(h, e, "particle download");
-    return sync(h);
+    const int rc = sync(h);
+    if (rc != SPH_OK) return rc;
+    std::memcpy(out, h->host_pinned, cnt * sizeof(float));
+    return SPH_OK;
 }
 
 int sph_get_particles(sph_handle* h, float* xyz_out) { return fetch3(h, h ? h->pos.p : nullptr, xyz_out); }
@@ -639,6 +681,24 @@ int sph_get_density(sph_handle* h, float* rho_out) {
     if (e == hipSuccess) e = hipMemcpyAsync(rho_out, h->staging.p, (size_t)h->n * sizeof(float), hipMemcpyDeviceToHost, h->stream);
     if (e != hipSuccess) return hip_fail(h, e, "density download");
     return sync(h);
+}
+
+int sph_export_frame(sph_handle* h, const char* path) {
+    if (!h || !path) return SPH_E_INVALID;
+    if (!h->has_particles) return fail(h, SPH_E_NOT_READY, "no particles set");
+    const size_t n = h->n;
+    std::vector<float> x(3 * n), v(3 * n), xv(6 * n);
+    int rc = sph_get_particles(h, x.data());
+    if (rc == SPH_OK) rc = sph_get_velocity(h, v.data());
+    if (rc != SPH_OK) return rc;
+    for (size_t i = 0; i < n; ++i)
+        for (int c = 0; c < 3; ++c) {
+            xv[6 * i + c] = x[3 * i + c];
+            xv[6 * i + 3 + c] = v[3 * i + c];
+        }
+    if (!write_npy_f32(path, xv.data(), (uint32_t)n, 6u))
+        return fail(h, SPH_E_INVALID, std::string("cannot write ") + path);
+    return SPH_OK;
 }
 
 int sph_synchronize(sph_handle* h) {

@@ -110,6 +110,8 @@ struct sph_handle {
     std::map<int, sph::DevBuf<uint2>> tiles;
     std::map<int, uint32_t> ntiles;
     sph::DevBuf<float> staging;
+    float* host_pinned = nullptr;          // 3n floats of page-locked host memory for downloads
+    size_t host_pinned_n = 0;
 
     // Graph of one step.
     bool use_graph = true;

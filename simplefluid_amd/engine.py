@@ -46,7 +46,7 @@ EXPORTS = (
     "sph_params_default", "sph_params_set_resolution", "sph_create", "sph_destroy",
     "sph_set_params", "sph_get_params", "sph_set_particles", "sph_make_ready",
     "sph_advance_frame", "sph_advance_steps", "sph_run_frame", "sph_num_particles",
-    "sph_get_particles", "sph_get_velocity", "sph_get_density", "sph_synchronize",
+    "sph_get_particles", "sph_get_velocity", "sph_get_density", "sph_export_frame", "sph_synchronize",
     "sph_last_error", "sph_set_use_graph", "sph_scene_generate", "sph_batch_create",
     "sph_num_scenes", "sph_scene_offsets", "sph_set_scene_params", "sph_rollout_forward",
     "sph_rollout_backward", "sph_profile_enable", "sph_profile_read", "sph_autotune",
@@ -119,6 +119,7 @@ def load_library(path: str = LIB_PATH):
         "sph_get_particles": (i32, [H, f32p]),
         "sph_get_velocity": (i32, [H, f32p]),
         "sph_get_density": (i32, [H, f32p]),
+        "sph_export_frame": (i32, [H, C.c_char_p]),
         "sph_synchronize": (i32, [H]),
         "sph_last_error": (C.c_char_p, [H]),
         "sph_set_use_graph": (i32, [H, i32]),
@@ -201,6 +202,14 @@ def make_params(**kw) -> sph_params:
                 box_max=(1.0, 1.0, 1.0), stop_time=5.0, num_threads=0, cfl_number=0.0)
     base.update(kw)
     return set_fields(p, **base)
+
+
+def load_frame(path: str):
+    """(positions, velocities) of a frame written by export_frame."""
+    a = np.load(path, allow_pickle=False)
+    if a.ndim != 2 or a.shape[1] != 6 or a.dtype != np.float32:
+        raise ValueError(f"{path}: not an engine frame (shape {a.shape}, {a.dtype})")
+    return np.ascontiguousarray(a[:, :3]), np.ascontiguousarray(a[:, 3:])
 
 
 def scene_generate(scene: int, resolution: int = 24) -> np.ndarray:
@@ -292,6 +301,10 @@ class Engine:
         out = np.empty((self.num_particles, 3), np.float32)
         self._chk(self._L.sph_get_velocity(self._h, _p(out)))
         return out
+
+    def export_frame(self, path: str):
+        """Writes the state as a (n, 6) float32 .npy: x, y, z, vx, vy, vz in original order."""
+        self._chk(self._L.sph_export_frame(self._h, os.fsencode(path)))
 
     def get_density(self) -> np.ndarray:
         out = np.empty(self.num_particles, np.float32)

@@ -1,5 +1,9 @@
-"""Writes tests/golden/tiny_step.npz: one FP64 oracle step of a small jittered
-lattice (inputs, permutation, neighbour CSR, density, acceleration).
+"""Writes the golden fixtures of tests/:
+  tiny_step.npz  one FP64 oracle step of a small jittered lattice (inputs,
+                 permutation, neighbour CSR, density, acceleration);
+  frame_cubedrop16_step40.npy  the reference CubeDrop at resolution 16 with
+                 the GUI defaults after 40 FP64 oracle steps, in the engine's
+                 frame-export format (float32 (n, 6): x, y, z, vx, vy, vz).
 
 The reference (ttnghia/SimpleFluid) holds no vectors for its solver, which is
 not vendored; these fixtures freeze the restatement once it passed the
@@ -35,6 +39,19 @@ def main():
     np.savez(out, pos=pos, vel=vel, B=B, mu=mu, perm=d["perm"], nbr_off=off, nbr_list=lst,
              rho=d["rho"], acc=d["acc"])
     print("wrote", out, len(pos), "particles,", len(lst), "neighbour entries")
+
+    # Frame fixture: host-only calls of the engine library (parameters and the
+    # scene generator need no GPU), stepped by the oracle.
+    from simplefluid_amd import engine
+    p = engine.default_params(16)
+    x0 = engine.scene_generate(engine.SCENE_CUBE_DROP, 16)
+    o = oracle.Oracle(oracle.from_params(p, 4), fp64=True, threads=8)
+    o.set_state(x0.astype(np.float64))
+    o.step(40)
+    x, v = o.state_by_id()
+    out = os.path.join(os.path.dirname(os.path.abspath(__file__)), "frame_cubedrop16_step40.npy")
+    np.save(out, np.concatenate([x, v], 1).astype(np.float32))
+    print("wrote", out, x.shape[0], "particles")
 
 
 if __name__ == "__main__":

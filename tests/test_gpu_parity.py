@@ -275,6 +275,27 @@ def test_cfl_step_parity(E):
         E.BatchEngine([p, p], [10, 10])     # a batch shares one dt
 
 
+def test_frame_export_matches_golden(E, tmp_path):
+    """sph_export_frame writes the state as (n, 6) float32 .npy in original
+    order; after 40 steps of the reference CubeDrop (resolution 16, GUI
+    defaults) it matches the FP64 oracle fixture to FP32 round-off."""
+    import os
+    p = E.default_params(16)
+    e = make_engine(E, p, E.scene_generate(E.SCENE_CUBE_DROP, 16))
+    e.set_cell_subdivision(4)
+    e.advance_steps(40)
+    path = str(tmp_path / "frame.npy")
+    e.export_frame(path)
+    x, v = E.load_frame(path)
+    assert np.array_equal(x, e.get_particles()) and np.array_equal(v, e.get_velocity())
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    xg, vg = E.load_frame(os.path.join(root, "tests", "golden", "frame_cubedrop16_step40.npy"))
+    say(f"[frame] pos err {np.abs(x - xg).max():.2e} vel err {np.abs(v - vg).max():.2e}")
+    assert np.abs(x - xg).max() < 2e-6
+    assert np.abs(v - vg).max() < 1e-4
+    e.close()
+
+
 def test_get_particles_original_order(E):
     from simplefluid_amd import configs
     cfg = configs.dambreak_65k()

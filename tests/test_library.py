@@ -116,3 +116,19 @@ def test_no_gpu_fails_loudly(lib):
         engine.Engine(engine.make_params())
     assert ei.value.code == engine.SPH_E_HIP
     assert "HIP device" in str(ei.value)
+
+
+def test_golden_frame_fixture_loads_and_matches_fp32_oracle(lib, oracle_lib):
+    """The committed frame (export format) loads with the binding's reader and
+    an FP32 oracle rollout of the same scene lands within FP32 round-off of it."""
+    import oracle
+    from simplefluid_amd import engine
+    x, v = engine.load_frame(os.path.join(ROOT, "tests", "golden", "frame_cubedrop16_step40.npy"))
+    assert x.shape == (4096, 3) and v.shape == (4096, 3)
+    p = engine.default_params(16)
+    o = oracle.Oracle(oracle.from_params(p, 4), fp64=False, threads=8)
+    o.set_state(engine.scene_generate(engine.SCENE_CUBE_DROP, 16).astype(np.float64))
+    o.step(40)
+    xo, vo = o.state_by_id()
+    assert np.abs(xo - x).max() < 2e-6
+    assert np.abs(vo - v).max() < 1e-4
