@@ -32,6 +32,8 @@ sys.path.insert(0, ROOT)
 
 METRIC = "particle-steps/s @1M (plain vs batched); % HBM roofline; adjoint/fwd ratio"
 HBM_PEAK_GBS = 8000.0
+# Non-packed FP32 VALU issue: 256 CUs x 4 SIMD x 16 lanes x 2.4 GHz.
+VALU_PEAK_TOPS = 39.32
 # Algorithmic bytes per particle and pass (FP32, unpadded float3): hash 20,
 # sort 16 (one ideal pass), reorder 52, cell table 4 + 8 per occupied cell,
 # density 16, force 40, integrate + clamp 60.  The engine fuses hash into the
@@ -92,6 +94,13 @@ def plain(args):
     roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic}
+    valu = load_valu(dom)
+    if valu:
+        # Lane operations per launch (rocprofv3 SQ_INSTS_VALU x 64) over the
+        # live mean duration: the issue-rate ceiling these kernels meet first.
+        lane_ops = valu * 64.0 / (per[dom] * 1e-3)
+        roof["valu"] = {"achieved": round(lane_ops / 1e12, 2), "peak": VALU_PEAK_TOPS, "unit": "T lane-op/s",
+                        "frac": round(lane_ops / 1e12 / VALU_PEAK_TOPS, 4)}
     step_ms = t / args.steps * 1e3
     step_roof = b_step * value / 1e9
     extra = {"step_bytes_alg_per_particle": round(b_step, 2),
@@ -102,6 +111,17 @@ def plain(args):
              "download_ms": round(download_ms, 3)}
     e.close()
     return value, step_ms, roof, extra, cfg
+
+
+def load_valu(kernel):
+    """VALU instructions per launch (wave-instructions) from the committed profile."""
+    path = os.path.join(ROOT, "profiles", "valu.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        return json.load(open(path)).get(kernel, {}).get("valu_insts_per_launch")
+    except Exception:
+        return None
 
 
 def load_traffic(kernel):
@@ -156,6 +176,9 @@ def batched(args):
     tb = time_steps(b, k)
     vb = N * k / tb
     b.close()
+    if args.no_per_scene:
+        log(f"batched: {N} particles in 256 scenes, {k} steps: {vb:.4e}")
+        return {"scenes": 256, "particles": N, "steps": k, "value": round(vb, 1), "unit": "particle-steps/s"}
     # Per-scene launches: one engine per scene, stepped scene by scene.
     engines = []
     for s in ens["scenes"]:
@@ -266,6 +289,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--batch-steps", type=int, default=100)
     ap.add_argument("--sphere-steps", type=int, default=100)
+    ap.add_argument("--no-per-scene", action="store_true",
+                    help="batched leg without the per-scene comparison (profiling)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")

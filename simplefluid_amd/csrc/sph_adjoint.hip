@@ -182,20 +182,49 @@ __global__ void adj_density_kernel(AdjArgs a) {
 
 // Per-scene sum of a per-particle array in a fixed order (one block per
 // scene), added to a double accumulator: deterministic run to run.
-__global__ __launch_bounds__(256) void scene_reduce_kernel(const SceneConst* __restrict__ scenes,
-                                                           const float* __restrict__ part,
-                                                           double* __restrict__ acc) {
-    __shared__ double s[256];
-    const SceneConst& sc = scenes[blockIdx.x];
-    double v = 0.0;
-    for (uint32_t i = sc.p_begin + threadIdx.x; i < sc.p_end; i += 256) v += (double)part[i];
-    s[threadIdx.x] = v;
+// Per-scene sums of two per-particle arrays in double, in a fixed order:
+// block (c, s) sums particles [p_begin + c 4096, + 4096) of scene s (threads
+// stride, then a tree); scene_total_kernel adds a scene's blocks in order.
+constexpr uint32_t kSumChunk = 4096;
+
+__global__ __launch_bounds__(256) void scene_partial_kernel(const SceneConst* __restrict__ scenes,
+                                                            const float* __restrict__ pa,
+                                                            const float* __restrict__ pb,
+                                                            double2* __restrict__ partial, uint32_t chunks) {
+    __shared__ double sa[256], sb[256];
+    const SceneConst& sc = scenes[blockIdx.y];
+    const uint32_t beg = sc.p_begin + blockIdx.x * kSumChunk;
+    const uint32_t end = beg + kSumChunk < sc.p_end ? beg + kSumChunk : sc.p_end;
+    double va = 0.0, vb = 0.0;
+    for (uint32_t i = beg + threadIdx.x; i < end; i += 256) {
+        va += (double)pa[i];
+        vb += (double)pb[i];
+    }
+    sa[threadIdx.x] = va;
+    sb[threadIdx.x] = vb;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+        if ((int)threadIdx.x < o) {
+            sa[threadIdx.x] += sa[threadIdx.x + o];
+            sb[threadIdx.x] += sb[threadIdx.x + o];
+        }
         __syncthreads();
     }
-    if (threadIdx.x == 0) acc[blockIdx.x] += s[0];
+    if (threadIdx.x == 0) partial[(size_t)blockIdx.y * chunks + blockIdx.x] = make_double2(sa[0], sb[0]);
+}
+
+__global__ void scene_total_kernel(uint32_t nscenes, const double2* __restrict__ partial, uint32_t chunks,
+                                   double* __restrict__ acc_a, double* __restrict__ acc_b) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nscenes) return;
+    double a = 0.0, b = 0.0;
+    for (uint32_t c = 0; c < chunks; ++c) {
+        const double2 v = partial[(size_t)s * chunks + c];
+        a += v.x;
+        b += v.y;
+    }
+    acc_a[s] += a;
+    acc_b[s] += b;
 }
 
 __global__ void gather3_kernel(uint32_t n, const float* __restrict__ g, const uint32_t* __restrict__ id,
@@ -226,10 +255,17 @@ hipError_t launch_adj_density(hipStream_t st, int block, uint32_t grid, const Ad
     return hipGetLastError();
 }
 
-hipError_t launch_scene_reduce(hipStream_t st, uint32_t nscenes, const SceneConst* scenes, const float* part,
-                               double* acc) {
-    if (nscenes == 0) return hipSuccess;
-    hipLaunchKernelGGL(scene_reduce_kernel, dim3(nscenes), dim3(256), 0, st, scenes, part, acc);
+uint32_t scene_sum_chunks(uint32_t max_scene_particles) {
+    return (max_scene_particles + kSumChunk - 1) / kSumChunk;
+}
+
+hipError_t launch_scene_sums(hipStream_t st, uint32_t nscenes, uint32_t chunks, const SceneConst* scenes,
+                             const float* pa, const float* pb, double2* partial, double* acc_a, double* acc_b) {
+    if (nscenes == 0 || chunks == 0) return hipSuccess;
+    hipLaunchKernelGGL(scene_partial_kernel, dim3(chunks, nscenes), dim3(256), 0, st, scenes, pa, pb, partial,
+                       chunks);
+    hipLaunchKernelGGL(scene_total_kernel, dim3((nscenes + 63) / 64), dim3(64), 0, st, nscenes, partial, chunks,
+                       acc_a, acc_b);
     return hipGetLastError();
 }
 

@@ -71,6 +71,7 @@ struct RolloutState {
     DevBuf<float4> xs, vs, abar;                   // per-step adjoint temporaries (sorted order)
     DevBuf<float> rhobar, bpart, mpart;
     DevBuf<double> gB, gmu;                        // per-scene accumulators
+    DevBuf<double2> gpart;                         // per-scene, per-chunk partial sums
     DevBuf<float> gbuf;                            // 3n staging for gradients
 };
 

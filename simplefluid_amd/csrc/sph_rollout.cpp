@@ -87,6 +87,10 @@ int sph_rollout_backward(sph_handle* h, const float* dL_dx_final, float* dL_dB, 
     chk(R.xs.ensure(n)); chk(R.vs.ensure(n)); chk(R.abar.ensure(n));
     chk(R.rhobar.ensure(n)); chk(R.bpart.ensure(n)); chk(R.mpart.ensure(n));
     chk(R.gB.ensure(ns)); chk(R.gmu.ensure(ns));
+    uint32_t max_scene = 0;
+    for (uint32_t s = 0; s < ns; ++s) max_scene = std::max(max_scene, h->sc_h[s].p_end - h->sc_h[s].p_begin);
+    const uint32_t chunks = scene_sum_chunks(max_scene);
+    chk(R.gpart.ensure((size_t)ns * std::max(chunks, 1u)));
     chk(R.gbuf.ensure(3 * n + 1));
     if (e != hipSuccess) return hip_fail(h, e, "adjoint allocation");
 
@@ -120,8 +124,7 @@ int sph_rollout_backward(sph_handle* h, const float* dL_dx_final, float* dL_dB, 
             a.scenes = h->sc_d.p; a.tiles = tiles; a.ntiles = ntiles; a.n = (uint32_t)n; a.ncells = h->ncells;
             chk(launch_adj_integrate(st, 256, grid, a));
             chk(launch_adj_force(st, 256, grid, a));
-            chk(launch_scene_reduce(st, ns, h->sc_d.p, R.bpart.p, R.gB.p));
-            chk(launch_scene_reduce(st, ns, h->sc_d.p, R.mpart.p, R.gmu.p));
+            chk(launch_scene_sums(st, ns, chunks, h->sc_d.p, R.bpart.p, R.mpart.p, R.gpart.p, R.gB.p, R.gmu.p));
             chk(launch_adj_density(st, 256, grid, a));
             if (e != hipSuccess) return hip_fail(h, e, "adjoint launch");
         }

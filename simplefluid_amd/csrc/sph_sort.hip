@@ -102,28 +102,43 @@ constexpr uint32_t kInlineTiles = 1024;
 
 __global__ __launch_bounds__(kTileScanBlock) void tile_scan_kernel(uint32_t* __restrict__ count, uint32_t C,
                                                                    uint32_t T) {
+    // Rounds of 4096 tile sums: thread t takes 4 consecutive ones (16-byte
+    // access when aligned), a block scan adds the round's carry.
     __shared__ uint32_t s_w[kTileScanBlock / 64];
+    __shared__ uint32_t s_carry;
     const uint32_t* tsum = count + C;
     uint32_t* tpre = count + C + T;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const uint32_t per = (T + kTileScanBlock - 1) / kTileScanBlock;
-    const uint32_t i0 = (uint32_t)tid * per;
-    const uint32_t i1 = i0 + per < T ? i0 + per : T;
-    uint32_t own = 0;
-    for (uint32_t i = i0; i < i1; ++i) own += tsum[i];
-    uint32_t x = own;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t y = __shfl_up(x, o, 64);
-        if (lane >= o) x += y;
-    }
-    if (lane == 63) s_w[wave] = x;
+    if (tid == 0) s_carry = 0u;
     __syncthreads();
-    uint32_t run = x - own;
-    for (int w = 0; w < wave; ++w) run += s_w[w];
-    for (uint32_t i = i0; i < i1; ++i) {
-        tpre[i] = run;
-        run += tsum[i];
+    for (uint32_t r0 = 0; r0 < T; r0 += 4u * kTileScanBlock) {
+        const uint32_t i0 = r0 + 4u * (uint32_t)tid;
+        uint32_t v[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) v[e] = i0 + e < T ? tsum[i0 + e] : 0u;
+        const uint32_t own = v[0] + v[1] + v[2] + v[3];
+        uint32_t x = own;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(x, o, 64);
+            if (lane >= o) x += y;
+        }
+        if (lane == 63) s_w[wave] = x;
+        __syncthreads();
+        uint32_t run = s_carry + x - own;
+        uint32_t tot = 0;
+        for (int w = 0; w < kTileScanBlock / 64; ++w) {
+            if (w < wave) run += s_w[w];
+            tot += s_w[w];
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            if (i0 + e < T) tpre[i0 + e] = run;
+            run += v[e];
+        }
+        __syncthreads();
+        if (tid == 0) s_carry += tot;
+        __syncthreads();
     }
 }
 
