@@ -114,6 +114,54 @@ def plain(args):
     return value, step_ms, roof, extra, cfg
 
 
+def copy_ceiling_gbs():
+    """Achieved HBM bandwidth of a plain 256 MiB device-to-device copy (read +
+    write bytes over the best of 10 copies timed with HIP events): the
+    practical ceiling next to the 8 TB/s peak.  Uses the HIP runtime the
+    engine library loaded."""
+    import ctypes as C
+    from simplefluid_amd import engine
+    engine.load_library()
+    hip = C.CDLL("libamdhip64.so")
+    vp = C.c_void_p
+    nbytes = 256 << 20
+    a, b = vp(), vp()
+    e0, e1 = vp(), vp()
+    if hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) or hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)):
+        raise RuntimeError("hipMalloc failed")
+    try:
+        hip.hipEventCreate(C.byref(e0))
+        hip.hipEventCreate(C.byref(e1))
+        hip.hipMemset(a, 1, C.c_size_t(nbytes))
+        best = None
+        for _ in range(12):
+            hip.hipEventRecord(e0, None)
+            hip.hipMemcpyAsync(b, a, C.c_size_t(nbytes), 3, None)  # hipMemcpyDeviceToDevice
+            hip.hipEventRecord(e1, None)
+            hip.hipEventSynchronize(e1)
+            ms = C.c_float()
+            hip.hipEventElapsedTime(C.byref(ms), e0, e1)
+            best = ms.value if best is None else min(best, ms.value)
+        hip.hipEventDestroy(e0)
+        hip.hipEventDestroy(e1)
+    finally:
+        hip.hipFree(a)
+        hip.hipFree(b)
+    return round(2 * nbytes / (best * 1e-3) / 1e9, 1)
+
+
+def host_info():
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"model": model, "logical_cpus": os.cpu_count()}
+
+
 def load_valu(kernel):
     """VALU instructions per launch (wave-instructions) from the committed profile."""
     path = os.path.join(ROOT, "profiles", "valu.json")
@@ -158,7 +206,8 @@ def cpu_baseline(cfg, budget_s):
             "kind": "port",
             "sample": f"oracle C++ FP32 step (OpenMP), {n} particles of the same workload, {steps} steps "
                       f"after 1 warm-up, {spent:.1f}s",
-            "single_thread": {"value": round(n * s1 / t1, 1), "steps": s1, "seconds": round(t1, 2)}}
+            "single_thread": {"value": round(n * s1 / t1, 1), "steps": s1, "seconds": round(t1, 2)},
+            "host": host_info()}
 
 
 def batched(args):
@@ -326,6 +375,10 @@ def main():
                       "box": [2.0, 1.0, 2.0], "particle_radius": 0.005, "kernel_radius": 0.02,
                       "dt": 1e-4, "stiffness": 50.0, "viscosity": 0.01},
            "roofline": roof, "step": extra}
+    try:
+        out["roofline"]["copy_ceiling_GBs"] = copy_ceiling_gbs()
+    except Exception as ex:
+        log("copy ceiling not measured:", ex)
     if not args.no_extras:
         out["batched"] = batched(args)
         out["adjoint"] = adjoint(args)

@@ -98,6 +98,8 @@ extern "C" int sph_autotune(sph_handle* h) {
                    {&best.reorder_block, blocks, 4, SPH_PASS_REORDER},
                    {&best.sort_items, items, 4, SPH_PASS_SORT}};
     for (const Axis& ax : axes) {
+        // Keys per thread only shape the radix sort's digit passes.
+        if (ax.field == &best.sort_items && best.sort_mode != kSortRadix) continue;
         double best_ms = 1e30;
         int best_v = *ax.field;
         for (int c = 0; c < ax.ncand; ++c) {
