@@ -1,15 +1,18 @@
 // Shared device-side definitions of the gfx950 SPH engine.
 //
 // Data layout in HBM (N particles, C cells over all scenes):
-//   pos[2], vel[2]  float4 x N   state order / sorted order (w unused, 16 B aligned loads)
+//   pos[2], vel[2]  float4 x N   state order / sorted order (16 B aligned loads); in
+//                                sorted order pos.w = p/rho^2 and vel.w = 1/rho
+//                                (written by the density pass for the force pass)
 //   id[2]           uint32 x N   original particle id, state / sorted order
-//   key0            uint32 x N   cell key of the state positions (written by integrate)
-//   sort ping-pong  2 x (uint32 key + uint32 value) x N
-//   count           uint32 x C+T particles per cell, then per scan tile of 4096 cells
-//                                (atomic, built by the integrate pass)
+//   key0, rank0     uint32 x N   cell key of the state positions and the slot taken
+//                                in its cell count (written by integrate)
+//   sort ping-pong  2 x (uint32 key + uint32 value) x N (radix mode)
+//   count           uint32 x C+2T particles per cell, then per scan tile of 4096 cells
+//                                (atomic, built by the integrate pass), then the
+//                                exclusive scan of the tile sums
 //   cstart          uint32 x C+1 lower-bound cell-start table: cstart[c] = #particles with key < c
 //   rho             float  x N   density (sorted order)
-//   aux             float2 x N   (p/rho^2, 1/rho) read by the force pass
 //   mask            uint32 x 9N  neighbour bits, row-major [row][particle] (layout at
 //                                mask_span below)
 // All kernels are written for wave64 and 256 CUs in 8 XCDs.

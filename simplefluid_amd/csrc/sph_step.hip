@@ -5,16 +5,17 @@
 //   force    pressure + viscosity from the masks, integrate, clamp, next key
 // (sorting and the gather into sorted order live in sph_sort.hip).
 // Neighbour order is canonical: rows (dz, dy) = (-1..1, -1..1), dz outer; a
-// row is the contiguous sorted range of cells cx-1..cx+1; ascending index.
-// Density sums and force sums run in that order one particle per lane.
+// row is the contiguous sorted range of the x cells inside the support
+// sphere's chord at that row; ascending index.  Density sums and force sums
+// run in that order, one particle per lane.
 #include "sph_internal.hpp"
 
 namespace sph {
 
 namespace {
 
-// Tiles handled by this block: a contiguous run per block, and contiguous
-// blocks per XCD (persistent launches use gridDim < ntiles).
+// Tiles handled by this block (hash and the debug neighbour dump): a
+// contiguous run per block when gridDim < ntiles, XCD-aware order.
 __device__ __forceinline__ void tile_span(uint32_t ntiles, uint32_t& t0, uint32_t& t1) {
     const uint32_t lb = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t per = (ntiles + gridDim.x - 1) / gridDim.x;
