@@ -156,8 +156,10 @@ __global__ __launch_bounds__(BLOCK) void density_kernel(StepArgs a) {
     });
 }
 
+// Eight waves per SIMD (64 VGPRs, a 16-byte spill): measured 4 % faster than
+// the seven the register count would otherwise allow.
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void force_kernel(StepArgs a) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8, 8))) void force_kernel(StepArgs a) {
     const __amdgpu_buffer_rsrc_t rpos = buffer_of(a.pos_s, a.n);
     const __amdgpu_buffer_rsrc_t rvel = buffer_of(a.vel_s, a.n);
     const __amdgpu_buffer_rsrc_t rcs = buffer_of(a.cstart, a.ncells + 1u);
