@@ -159,7 +159,9 @@ def host_info():
                 break
     except OSError:
         pass
-    return {"model": model, "logical_cpus": os.cpu_count()}
+    import glob
+    return {"model": model, "logical_cpus": os.cpu_count(),
+            "numa_nodes": len(glob.glob("/sys/devices/system/node/node[0-9]*"))}
 
 
 def load_valu(kernel):
@@ -186,27 +188,33 @@ def load_traffic(kernel):
 
 
 def cpu_baseline(cfg, budget_s):
+    """The oracle's OpenMP FP32 step on this host: median of three timed
+    chunks after warm-up steps, with all host threads and with one thread,
+    on the 1M workload, plus a sample of the 65k dam break's 1000 steps."""
     import oracle
+    from simplefluid_amd import configs
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+
+    def run(c, nthreads, budget, max_steps, warm):
+        o = oracle.Oracle(oracle.from_params(c["params"], 4), threads=nthreads)
+        o.set_state(c["pos"].astype(np.float64))
+        o.step(warm)
+        first = o.step(1)  # per-step estimate sizes the chunks
+        k = int(max(1, min(max_steps, budget / 3.0 / max(first, 1e-9))))
+        rates = sorted(c["pos"].shape[0] * k / o.step(k) for _ in range(3))
+        return rates[1], k
+
     n = cfg["pos"].shape[0]
-
-    def run(nthreads, budget, max_steps):
-        o = oracle.Oracle(oracle.from_params(cfg["params"], 4), threads=nthreads)
-        o.set_state(cfg["pos"].astype(np.float64))
-        o.step(1)  # warm-up
-        steps, spent = 0, 0.0
-        while spent < budget and steps < max_steps:
-            spent += o.step(1)
-            steps += 1
-        return steps, spent
-
-    steps, spent = run(threads, budget_s, 50)
-    s1, t1 = run(1, budget_s / 3.0, 5)
-    return {"value": round(n * steps / spent, 1), "unit": "particle-steps/s", "cores": threads,
-            "kind": "port",
-            "sample": f"oracle C++ FP32 step (OpenMP), {n} particles of the same workload, {steps} steps "
-                      f"after 1 warm-up, {spent:.1f}s",
-            "single_thread": {"value": round(n * s1 / t1, 1), "steps": s1, "seconds": round(t1, 2)},
+    v, k = run(cfg, threads, budget_s, 20, 2)
+    v1, k1 = run(cfg, 1, budget_s / 3.0, 3, 1)
+    small = configs.dambreak_65k()
+    vs, ks = run(small, threads, budget_s / 3.0, 300, 10)
+    return {"value": round(v, 1), "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": f"oracle C++ FP32 step (OpenMP, {threads} threads), {n} particles of the same workload: "
+                      f"median of 3 chunks of {k} steps after 3 warm-up steps",
+            "single_thread": {"value": round(v1, 1), "sample": f"median of 3 chunks of {k1} steps"},
+            "dambreak_65k": {"value": round(vs, 1), "threads": threads,
+                             "sample": f"median of 3 chunks of {ks} of its 1000 steps after 11 warm-up steps"},
             "host": host_info()}
 
 
