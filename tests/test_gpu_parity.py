@@ -441,6 +441,27 @@ def test_batched_equals_per_scene(E):
         e.close()
 
 
+def test_batched_persistent_queues_match(E):
+    """A batch of six ~100k-particle scenes (more 64-lane tiles than resident
+    blocks, so the per-XCD tile queues are in use) gives the same bits with
+    persistent blocks as with one block per tile."""
+    from simplefluid_amd import configs
+    ens = configs.ensemble(nscenes=6, seed=12, nmin=90000, nmax=110000)
+    params = [s["params"] for s in ens["scenes"]]
+    counts = [s["pos"].shape[0] for s in ens["scenes"]]
+    assert sum(counts) // 64 > 256 * 32
+    out = []
+    for persistent in (0, 1):
+        b = E.BatchEngine(params, counts)
+        b.set_particles(np.concatenate([s["pos"] for s in ens["scenes"]]))
+        b.make_ready()
+        b.set_tuning(64, 64, 256, 8, persistent)
+        b.advance_steps(15)
+        out.append((b.get_particles(), b.get_velocity()))
+        b.close()
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
 def test_batched_parity_vs_oracle(E):
     """One batched step of 4 scenes vs the oracle run scene by scene."""
     import oracle
