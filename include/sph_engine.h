@@ -113,6 +113,16 @@ int  sph_get_particles(sph_handle* h, float* xyz_out);
 int  sph_get_velocity(sph_handle* h, float* vel_out);
 int  sph_get_density(sph_handle* h, float* rho_out);   /* density of the last step */
 int  sph_synchronize(sph_handle* h);
+/* Frame-cadence position snapshot that overlaps the PCIe copy with further
+ * steps (no reference counterpart: the reference's renderer reads the
+ * solver's array while the worker writes it, Source/Simulator.cpp:50 ->
+ * Source/FluidRenderWidget.cpp:211).  begin() captures the current positions
+ * (original order) in stream order and returns at once; steps enqueued after
+ * it run while a second stream copies the capture to page-locked memory;
+ * wait() blocks until that copy is done and writes 3n floats.  One snapshot
+ * is in flight at a time; a begin() waits on the GPU for the previous copy. */
+int  sph_snapshot_begin(sph_handle* h);
+int  sph_snapshot_wait(sph_handle* h, float* xyz_out);
 /* State dump of one frame (no reference counterpart: the reference exports
  * only screenshots, Source/MainWindow.cpp:108-113): a NumPy .npy file of
  * float32, shape (n, 6) = x, y, z, vx, vy, vz per particle in original

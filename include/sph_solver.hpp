@@ -189,6 +189,14 @@ public:
         refresh();
         return velocities_;
     }
+    /* Frame-cadence hand-off to a renderer: snapshotBegin() captures the
+     * positions now and returns; later advanceFrame() calls overlap the copy
+     * to the host; snapshotWait() fills `out` with the captured positions. */
+    void snapshotBegin() { check(sph_snapshot_begin(h_)); }
+    void snapshotWait(Vec_Vec3& out) {
+        out.resize(sph_num_particles(h_));
+        check(sph_snapshot_wait(h_, reinterpret_cast<float*>(out.data())));
+    }
     /* State dump of the current frame (.npy, float32 (n, 6): x, v in original order). */
     void exportFrame(const std::string& path) { check(sph_export_frame(h_, path.c_str())); }
     std::shared_ptr<SPHParameters> getParameters() const { return params_; }

@@ -10,6 +10,11 @@
 using namespace sph;
 
 sph_handle::~sph_handle() {
+    if (copy_stream) (void)hipStreamSynchronize(copy_stream);
+    if (snap_host) (void)hipHostFree(snap_host);
+    if (snap_ready) (void)hipEventDestroy(snap_ready);
+    if (snap_done) (void)hipEventDestroy(snap_done);
+    if (copy_stream) (void)hipStreamDestroy(copy_stream);
     if (host_pinned) (void)hipHostFree(host_pinned);
     if (gexec) (void)hipGraphExecDestroy(gexec);
     if (graph) (void)hipGraphDestroy(graph);
@@ -681,6 +686,53 @@ int sph_get_density(sph_handle* h, float* rho_out) {
     if (e == hipSuccess) e = hipMemcpyAsync(rho_out, h->staging.p, (size_t)h->n * sizeof(float), hipMemcpyDeviceToHost, h->stream);
     if (e != hipSuccess) return hip_fail(h, e, "density download");
     return sync(h);
+}
+
+int sph_snapshot_begin(sph_handle* h) {
+    if (!h) return SPH_E_INVALID;
+    if (!h->has_particles || h->n == 0) return fail(h, SPH_E_NOT_READY, "no particles set");
+    const size_t cnt = 3 * (size_t)h->n;
+    hipError_t e = hipSuccess;
+    if (!h->copy_stream) e = hipStreamCreateWithFlags(&h->copy_stream, hipStreamNonBlocking);
+    if (e == hipSuccess && !h->snap_ready) e = hipEventCreateWithFlags(&h->snap_ready, hipEventDisableTiming);
+    if (e == hipSuccess && !h->snap_done) e = hipEventCreateWithFlags(&h->snap_done, hipEventDisableTiming);
+    if (e != hipSuccess) return hip_fail(h, e, "snapshot stream");
+    if (h->snap_n < cnt) {
+        // A resize waits for any copy still reading the old buffers.
+        if (h->snap_pending) (void)hipEventSynchronize(h->snap_done);
+        h->snap_pending = false;
+        if (h->snap_host) (void)hipHostFree(h->snap_host);
+        h->snap_host = nullptr;
+        h->snap_n = 0;
+        e = h->snap_d.ensure(cnt);
+        if (e == hipSuccess) e = hipHostMalloc((void**)&h->snap_host, cnt * sizeof(float), hipHostMallocDefault);
+        if (e != hipSuccess) {
+            h->snap_host = nullptr;
+            return hip_fail(h, e, "snapshot buffers");
+        }
+        h->snap_n = cnt;
+    }
+    // The previous snapshot's copy must have left snap_d before it is rewritten.
+    if (h->snap_pending) e = hipStreamWaitEvent(h->stream, h->snap_done, 0);
+    if (e == hipSuccess) e = launch_unpermute3(h->stream, h->n, h->pos.p, h->id.p, h->snap_d.p);
+    if (e == hipSuccess) e = hipEventRecord(h->snap_ready, h->stream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(h->copy_stream, h->snap_ready, 0);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(h->snap_host, h->snap_d.p, cnt * sizeof(float), hipMemcpyDeviceToHost, h->copy_stream);
+    if (e == hipSuccess) e = hipEventRecord(h->snap_done, h->copy_stream);
+    if (e != hipSuccess) return hip_fail(h, e, "snapshot enqueue");
+    h->snap_pending = true;
+    return SPH_OK;
+}
+
+int sph_snapshot_wait(sph_handle* h, float* xyz_out) {
+    if (!h || !xyz_out) return SPH_E_INVALID;
+    if (!h->snap_pending) return fail(h, SPH_E_NOT_READY, "no snapshot in flight (call sph_snapshot_begin)");
+    const hipError_t e = hipEventSynchronize(h->snap_done);
+    if (e != hipSuccess) return hip_fail(h, e, "snapshot wait");
+    h->snap_pending = false;
+    std::memcpy(xyz_out, h->snap_host, 3 * (size_t)h->n * sizeof(float));
+    return check_device_error(h);
 }
 
 int sph_export_frame(sph_handle* h, const char* path) {

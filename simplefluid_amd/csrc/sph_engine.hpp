@@ -113,6 +113,14 @@ struct sph_handle {
     sph::DevBuf<float> staging;
     float* host_pinned = nullptr;          // 3n floats of page-locked host memory for downloads
     size_t host_pinned_n = 0;
+    // Asynchronous frame snapshot: positions un-permuted on the step stream
+    // into snap_d, copied to snap_host on copy_stream while steps continue.
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t snap_ready = nullptr, snap_done = nullptr;
+    sph::DevBuf<float> snap_d;
+    float* snap_host = nullptr;
+    size_t snap_n = 0;
+    bool snap_pending = false;
 
     // Graph of one step.
     bool use_graph = true;

@@ -46,7 +46,7 @@ EXPORTS = (
     "sph_params_default", "sph_params_set_resolution", "sph_create", "sph_destroy",
     "sph_set_params", "sph_get_params", "sph_set_particles", "sph_make_ready",
     "sph_advance_frame", "sph_advance_steps", "sph_run_frame", "sph_num_particles",
-    "sph_get_particles", "sph_get_velocity", "sph_get_density", "sph_export_frame", "sph_synchronize",
+    "sph_get_particles", "sph_get_velocity", "sph_get_density", "sph_export_frame", "sph_snapshot_begin", "sph_snapshot_wait", "sph_synchronize",
     "sph_last_error", "sph_set_use_graph", "sph_scene_generate", "sph_batch_create",
     "sph_num_scenes", "sph_scene_offsets", "sph_set_scene_params", "sph_rollout_forward",
     "sph_rollout_backward", "sph_profile_enable", "sph_profile_read", "sph_autotune",
@@ -119,6 +119,8 @@ def load_library(path: str = LIB_PATH):
         "sph_get_particles": (i32, [H, f32p]),
         "sph_get_velocity": (i32, [H, f32p]),
         "sph_get_density": (i32, [H, f32p]),
+        "sph_snapshot_begin": (i32, [H]),
+        "sph_snapshot_wait": (i32, [H, f32p]),
         "sph_export_frame": (i32, [H, C.c_char_p]),
         "sph_synchronize": (i32, [H]),
         "sph_last_error": (C.c_char_p, [H]),
@@ -300,6 +302,16 @@ class Engine:
     def get_velocity(self) -> np.ndarray:
         out = np.empty((self.num_particles, 3), np.float32)
         self._chk(self._L.sph_get_velocity(self._h, _p(out)))
+        return out
+
+    def snapshot_begin(self):
+        """Capture the positions now; the copy to the host overlaps later steps."""
+        self._chk(self._L.sph_snapshot_begin(self._h))
+
+    def snapshot_wait(self) -> np.ndarray:
+        """Positions (original order) captured by the last snapshot_begin."""
+        out = np.empty((self.num_particles, 3), np.float32)
+        self._chk(self._L.sph_snapshot_wait(self._h, _p(out)))
         return out
 
     def export_frame(self, path: str):

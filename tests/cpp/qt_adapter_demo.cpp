@@ -41,8 +41,14 @@ int main(int argc, char** argv) {
             sim_time += frame_time;
             frame_times.push_back(frame_time);
         }
+        // The renderer's hand-off: a snapshot taken now, collected later.
+        solver.snapshotBegin();
+        std::vector<sph::Vec3f> snap;
+        solver.snapshotWait(snap);
         const auto& xp = solver.getParticles();
         const auto& vp = solver.getVelocity();
+        for (size_t i = 0; i < snap.size(); ++i)
+            if (snap[i].x != xp[i].x || snap[i].y != xp[i].y || snap[i].z != xp[i].z) return 5;
         std::FILE* out = std::fopen(argv[4], "wb");
         if (!out) return 4;
         const uint32_t nf = (uint32_t)frame_times.size(), np = solver.getNumParticles();

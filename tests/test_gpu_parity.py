@@ -296,6 +296,30 @@ def test_frame_export_matches_golden(E, tmp_path):
     e.close()
 
 
+def test_snapshot_overlaps_later_steps(E):
+    """A snapshot holds the positions of the moment it began, whatever steps
+    run before it is collected; a second snapshot replaces the first."""
+    from simplefluid_amd import configs
+    cfg = configs.dambreak_65k(jitter=1e-6, seed=14)
+    e = make_engine(E, cfg["params"], cfg["pos"])
+    e.advance_steps(5)
+    x0 = e.get_particles()
+    e.snapshot_begin()
+    e.advance_steps(20)
+    s0 = e.snapshot_wait()
+    assert np.array_equal(s0, x0)
+    x1 = e.get_particles()
+    assert not np.array_equal(x1, x0)
+    e.snapshot_begin()
+    e.advance_steps(3)
+    e.snapshot_begin()                     # waits for the first copy on the GPU
+    x2 = e.get_particles()
+    assert np.array_equal(e.snapshot_wait(), x2)
+    with pytest.raises(E.SPHError):
+        e.snapshot_wait()                  # nothing in flight
+    e.close()
+
+
 def test_get_particles_original_order(E):
     from simplefluid_amd import configs
     cfg = configs.dambreak_65k()
