@@ -807,6 +807,8 @@ int sph_set_scene_params(sph_handle* h, uint32_t scene, const sph_params* p) {
                            std::memcmp(p->box_min, o.box_min, sizeof(o.box_min)) == 0 &&
                            std::memcmp(p->box_max, o.box_max, sizeof(o.box_max)) == 0;
     if (!same_geom) return fail(h, SPH_E_INVALID, "box, radii, rest density and dt are fixed at create time");
+    if (h->batched && p->cfl_number > 0.f)
+        return fail(h, SPH_E_INVALID, "the CFL step is for single-scene handles (a batch shares dt)");
     SceneConst sc;
     std::string err;
     int rc = derive_scene(*p, sc, err, h->xsub);
