@@ -150,6 +150,7 @@ StepArgs step_args(sph_handle* h, StepBufs& w, int block) {
     StepArgs a;
     std::memset(&a, 0, sizeof(a));
     a.pos_s = w.pos_s.p;
+    a.soa = w.soa.p;
     a.vel_s = w.vel_s.p;
     a.pos_o = h->pos.p;
     a.vel_o = h->vel.p;
@@ -174,6 +175,7 @@ int alloc_step_bufs(sph_handle* h, StepBufs& w) {
     hipError_t e = hipSuccess;
     auto chk = [&](hipError_t x) { if (x != hipSuccess && e == hipSuccess) e = x; };
     chk(w.pos_s.ensure(n));
+    chk(w.soa.ensure(3 * n));
     chk(w.vel_s.ensure(n));
     chk(w.id_s.ensure(n));
     chk(w.skey.ensure(n));
@@ -272,6 +274,7 @@ int enqueue_step(sph_handle* h, StepBufs& w, bool record_prof) {
     sp.vel_in = h->vel.p;
     sp.id_in = h->id.p;
     sp.pos_out = w.pos_s.p;
+    sp.soa_out = w.soa.p;
     sp.vel_out = w.vel_s.p;
     sp.id_out = w.id_s.p;
     hipError_t e = launch_sort_keys(st, sp);

@@ -37,6 +37,7 @@ struct DevBuf {
 // checkpoint segment (its tape), the plain step uses slot 0.
 struct StepBufs {
     DevBuf<float4> pos_s, vel_s;   // sorted input state
+    DevBuf<float> soa;             // sorted positions as x, y, z planes (density candidates)
     DevBuf<uint32_t> id_s;
     DevBuf<uint32_t> skey, perm;   // sorted keys, sorted index -> state index
     DevBuf<uint32_t> cstart;       // C + 1 lower-bound cell-start table

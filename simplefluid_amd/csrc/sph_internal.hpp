@@ -38,6 +38,7 @@ struct SortPlan {
     const float4* vel_in = nullptr;
     const uint32_t* id_in = nullptr;
     float4* pos_out = nullptr;
+    float* soa_out = nullptr;         // sorted x, y, z planes (3 N floats) for the density pass
     float4* vel_out = nullptr;
     uint32_t* id_out = nullptr;
 };
@@ -56,6 +57,7 @@ hipError_t launch_sort_gather(hipStream_t st, const SortPlan& sp);
 // particles that never straddle two scenes (tiles[t] = (scene, first)).
 struct StepArgs {
     float4* pos_s;           // sorted positions; w <- p/rho^2 (density pass)
+    const float* soa;        // the same positions as x, y, z planes (3 N floats)
     float4* vel_s;           // sorted velocities; w <- 1/rho (density pass)
     float4* pos_o;           // state out (integrate)
     float4* vel_o;

@@ -266,7 +266,7 @@ __global__ void cell_rank_kernel(uint32_t n, uint32_t C, const uint32_t* __restr
                                  float4* __restrict__ vel_out, uint32_t* __restrict__ id_out,
                                  uint32_t* __restrict__ skey, uint32_t* __restrict__ perm,
                                  uint32_t* __restrict__ tsum, uint32_t tsum_words,
-                                 uint32_t* __restrict__ err) {
+                                 uint32_t* __restrict__ err, float* __restrict__ soa) {
     // The scan has read the tile sums; clear them for the integrate pass.
     if (blockIdx.x == 0)
         for (uint32_t i = threadIdx.x; i < tsum_words; i += blockDim.x) tsum[i] = 0u;
@@ -287,7 +287,11 @@ __global__ void cell_rank_kernel(uint32_t n, uint32_t C, const uint32_t* __restr
     }
     perm[dst] = k;
     skey[dst] = key;
-    pos_out[dst] = pos_in[k];
+    const float4 pk = pos_in[k];
+    pos_out[dst] = pk;
+    soa[dst] = pk.x;
+    soa[n + dst] = pk.y;
+    soa[2 * (size_t)n + dst] = pk.z;
     vel_out[dst] = vel_in[k];
     id_out[dst] = id_in[k];
 }
@@ -447,7 +451,7 @@ __global__ void radix_reorder_kernel(uint32_t n, const uint32_t* __restrict__ pe
                                      const uint32_t* __restrict__ id_in, float4* __restrict__ pos_out,
                                      float4* __restrict__ vel_out, uint32_t* __restrict__ id_out,
                                      uint32_t* __restrict__ hist_zero, uint32_t* __restrict__ tsum,
-                                     uint32_t tsum_words, uint32_t* __restrict__ err) {
+                                     uint32_t tsum_words, uint32_t* __restrict__ err, float* __restrict__ soa) {
     if (blockIdx.x == 0) {
         for (uint32_t i = threadIdx.x; i < (uint32_t)(kMaxPasses * kBins); i += blockDim.x) hist_zero[i] = 0u;
         for (uint32_t i = threadIdx.x; i < tsum_words; i += blockDim.x) tsum[i] = 0u;
@@ -459,7 +463,11 @@ __global__ void radix_reorder_kernel(uint32_t n, const uint32_t* __restrict__ pe
         atomicOr(err, 4u);
         return;
     }
-    pos_out[k] = pos_in[s];
+    const float4 ps = pos_in[s];
+    pos_out[k] = ps;
+    soa[k] = ps.x;
+    soa[n + k] = ps.y;
+    soa[2 * (size_t)n + k] = ps.z;
     vel_out[k] = vel_in[s];
     id_out[k] = id_in[s];
 }
@@ -537,11 +545,11 @@ hipError_t launch_sort_gather(hipStream_t st, const SortPlan& sp) {
         hipLaunchKernelGGL(cell_rank_kernel, dim3(blocks_for(sp.n, sp.reorder_block)), dim3(sp.reorder_block), 0,
                            st, sp.n, sp.ncells, sp.key0, sp.cstart, sp.tmp, sp.pos_in, sp.vel_in, sp.id_in,
                            sp.pos_out, sp.vel_out, sp.id_out, sp.skey, sp.perm, tsum,
-                           tsum_words, sp.err);
+                           tsum_words, sp.err, sp.soa_out);
     else
         hipLaunchKernelGGL(radix_reorder_kernel, dim3(blocks_for(sp.n, sp.reorder_block)), dim3(sp.reorder_block),
                            0, st, sp.n, sp.perm, sp.pos_in, sp.vel_in, sp.id_in, sp.pos_out, sp.vel_out,
-                           sp.id_out, sp.hist, tsum, tsum_words, sp.err);
+                           sp.id_out, sp.hist, tsum, tsum_words, sp.err, sp.soa_out);
     return hipGetLastError();
 }
 

@@ -23,8 +23,12 @@ __device__ __forceinline__ void tile_span(uint32_t ntiles, uint32_t& t0, uint32_
     t1 = t0 + per < ntiles ? t0 + per : ntiles;
 }
 
+// Candidates come from the x, y, z planes: one 16-byte load per plane gives
+// four consecutive candidates (three loads per trip instead of four, which
+// relieves the texture address units this pass otherwise saturates).
 __device__ __forceinline__ void density_one(const StepArgs& a, const SceneConst& sc, uint32_t k,
-                                            __amdgpu_buffer_rsrc_t rpos, __amdgpu_buffer_rsrc_t rcs) {
+                                            __amdgpu_buffer_rsrc_t rpos, __amdgpu_buffer_rsrc_t rsoa,
+                                            __amdgpu_buffer_rsrc_t rcs) {
     const float4 pi = load4(rpos, k << 4);
     const RowGeom g = row_geom(sc, pi.x, pi.y, pi.z);
     uint32_t lo[kRows], hi[kRows];
@@ -40,10 +44,11 @@ __device__ __forceinline__ void density_one(const StepArgs& a, const SceneConst&
         // row end shift in zero bits (past the array end the buffer returns 0).
         for (uint32_t j = l; j < e; j += 4) {
             const uint32_t rem = e - j;
-            const uint32_t off = j << 4;
-            float4 pj[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) pj[u] = load4(rpos, off + 16u * (uint32_t)u);
+            const float4 X = load4(rsoa, j << 2);
+            const float4 Y = load4(rsoa, (a.n + j) << 2);
+            const float4 Z = load4(rsoa, (2u * a.n + j) << 2);
+            const float4 pj[4] = {make_float4(X.x, Y.x, Z.x, 0.f), make_float4(X.y, Y.y, Z.y, 0.f),
+                                  make_float4(X.z, Y.z, Z.z, 0.f), make_float4(X.w, Y.w, Z.w, 0.f)};
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const float d2 = dist2(pi.x - pj[u].x, pi.y - pj[u].y, pi.z - pj[u].z);
@@ -147,12 +152,13 @@ __device__ __forceinline__ uint32_t force_one(const StepArgs& a, const SceneCons
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void density_kernel(StepArgs a) {
     const __amdgpu_buffer_rsrc_t rpos = buffer_of(a.pos_s, a.n);
+    const __amdgpu_buffer_rsrc_t rsoa = buffer_of(a.soa, 3u * a.n);
     const __amdgpu_buffer_rsrc_t rcs = buffer_of(a.cstart, a.ncells + 1u);
     for_each_tile(a.ntiles, a.queue, [&](uint32_t t) {
         const uint2 ti = a.tiles[t];
         const SceneConst& sc = a.scenes[ti.x];
         const uint32_t k = ti.y + threadIdx.x;
-        if (k < sc.p_end) density_one(a, sc, k, rpos, rcs);
+        if (k < sc.p_end) density_one(a, sc, k, rpos, rsoa, rcs);
     });
 }
 
