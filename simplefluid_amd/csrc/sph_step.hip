@@ -26,8 +26,28 @@ __device__ __forceinline__ void tile_span(uint32_t ntiles, uint32_t& t0, uint32_
 // Candidates come from the x, y, z planes: one 16-byte load per plane gives
 // four consecutive candidates (three loads per trip instead of four, which
 // relieves the texture address units this pass otherwise saturates).
+// Trips of four full candidates run without slot checks; the row's last,
+// partial trip masks the slots past its end.
+template <bool FULL>
+__device__ __forceinline__ void density_trip(const float4& pi, float h2, const float4& X, const float4& Y,
+                                             const float4& Z, int rem, float& sum, uint32_t& m) {
+    const float xs[4] = {X.x, X.y, X.z, X.w}, ys[4] = {Y.x, Y.y, Y.z, Y.w}, zs[4] = {Z.x, Z.y, Z.z, Z.w};
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        // q = d2 - h2 is negative exactly when d2 < h2 (an exact tie rounds
+        // to +0), so its sign bit is the mask bit and -q = h2 - d2 (rounding
+        // is symmetric) is the kernel's t; slots past the row end get q = 1.
+        float q = dist2(pi.x - xs[u], pi.y - ys[u], pi.z - zs[u]) - h2;
+        if (!FULL && u != 0 && u >= rem) q = 1.0f;
+        const float t = fmaxf(-q, 0.0f);
+        sum = fmaf(t * t, t, sum);
+        m = (m << 1) | (__float_as_uint(q) >> 31);
+    }
+}
+
 __device__ __forceinline__ void density_one(const StepArgs& a, const SceneConst& sc, uint32_t k,
-                                            __amdgpu_buffer_rsrc_t rpos, __amdgpu_buffer_rsrc_t rsoa,
+                                            __amdgpu_buffer_rsrc_t rpos, __amdgpu_buffer_rsrc_t rx,
+                                            __amdgpu_buffer_rsrc_t ry, __amdgpu_buffer_rsrc_t rz,
                                             __amdgpu_buffer_rsrc_t rcs) {
     const float4 pi = load4(rpos, k << 4);
     const RowGeom g = row_geom(sc, pi.x, pi.y, pi.z);
@@ -39,25 +59,15 @@ __device__ __forceinline__ void density_one(const StepArgs& a, const SceneConst&
 #pragma unroll
     for (int row = 0; row < kRows; ++row) {
         uint32_t m = 0u;
-        const uint32_t l = lo[row], e = hi[row];
-        // Four candidates per trip, loads in flight together; slots past the
-        // row end shift in zero bits (past the array end the buffer returns 0).
-        for (uint32_t j = l; j < e; j += 4) {
-            const uint32_t rem = e - j;
-            const float4 X = load4(rsoa, j << 2);
-            const float4 Y = load4(rsoa, (a.n + j) << 2);
-            const float4 Z = load4(rsoa, (2u * a.n + j) << 2);
-            const float4 pj[4] = {make_float4(X.x, Y.x, Z.x, 0.f), make_float4(X.y, Y.y, Z.y, 0.f),
-                                  make_float4(X.z, Y.z, Z.z, 0.f), make_float4(X.w, Y.w, Z.w, 0.f)};
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const float d2 = dist2(pi.x - pj[u].x, pi.y - pj[u].y, pi.z - pj[u].z);
-                const bool hit = (u == 0 || (uint32_t)u < rem) && d2 < h2;
-                const float t = hit ? h2 - d2 : 0.0f;
-                sum = fmaf(t * t, t, sum);
-                m = m + m + (hit ? 1u : 0u);
-            }
-        }
+        const uint32_t cnt = hi[row] - lo[row];
+        uint32_t off = lo[row] << 2;
+        const uint32_t full_end = off + ((cnt & ~3u) << 2);
+        // Slots past the row end shift in zero bits (past the array end the
+        // buffer returns 0).
+        for (; off != full_end; off += 16u)
+            density_trip<true>(pi, h2, load4(rx, off), load4(ry, off), load4(rz, off), 4, sum, m);
+        const int rem = (int)(cnt & 3u);
+        if (rem > 0) density_trip<false>(pi, h2, load4(rx, off), load4(ry, off), load4(rz, off), rem, sum, m);
         a.mask[(size_t)row * a.n + k] = m;
     }
     const float rho = sc.cD * sum;
@@ -152,13 +162,15 @@ __device__ __forceinline__ uint32_t force_one(const StepArgs& a, const SceneCons
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void density_kernel(StepArgs a) {
     const __amdgpu_buffer_rsrc_t rpos = buffer_of(a.pos_s, a.n);
-    const __amdgpu_buffer_rsrc_t rsoa = buffer_of(a.soa, 3u * a.n);
+    const __amdgpu_buffer_rsrc_t rx = buffer_of(a.soa, a.n);
+    const __amdgpu_buffer_rsrc_t ry = buffer_of(a.soa + a.n, a.n);
+    const __amdgpu_buffer_rsrc_t rz = buffer_of(a.soa + 2 * (size_t)a.n, a.n);
     const __amdgpu_buffer_rsrc_t rcs = buffer_of(a.cstart, a.ncells + 1u);
     for_each_tile(a.ntiles, a.queue, [&](uint32_t t) {
         const uint2 ti = a.tiles[t];
         const SceneConst& sc = a.scenes[ti.x];
         const uint32_t k = ti.y + threadIdx.x;
-        if (k < sc.p_end) density_one(a, sc, k, rpos, rsoa, rcs);
+        if (k < sc.p_end) density_one(a, sc, k, rpos, rx, ry, rz, rcs);
     });
 }
 
