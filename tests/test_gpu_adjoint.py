@@ -97,6 +97,31 @@ def test_adjoint_vs_fp32_oracle_and_checkpoint_invariance():
     assert np.linalg.norm(dv0 - odv0) <= GRAD_RTOL * np.linalg.norm(odv0)
 
 
+def test_adjoint_with_rows_longer_than_the_mask():
+    """A block compressed to spacing 1.25r beside a 2r block: rows of more
+    than 32 candidates, whose head the adjoint kernels re-test, in the same
+    waves as ordinary rows; against the FP32 reverse-mode oracle."""
+    import oracle
+    from simplefluid_amd import configs, engine
+    R = configs.R
+    rng = np.random.default_rng(21)
+    a = configs.lattice_block((8, 12, 8), (0.1, 0.0, 0.1))
+    i, j, k = np.meshgrid(np.arange(10), np.arange(12), np.arange(10), indexing="ij")
+    b = np.array([0.1 + 17 * R, R, 0.1 + R]) + 1.25 * R * np.stack([i.ravel(), j.ravel(), k.ravel()], 1)
+    pos = np.concatenate([a, b.astype(np.float32)])
+    vel = rng.normal(0, 0.1, pos.shape).astype(np.float32)
+    target = (pos + rng.normal(0, 0.003, pos.shape)).astype(np.float64)
+    p = engine.make_params(pressure_stiffness=50.0, viscosity=0.01, dt=1e-4)
+    steps = 6
+    _, oB, omu, odv0 = oracle.rollout_loss_grad(to_oracle(p), pos, vel, target, steps, fp64=False)
+    for xsub in (1, 4):
+        L, dB, dmu, dv0 = gpu_grads(p, pos, vel, target, steps, 3, xsub=xsub)[:4]
+        say(f"xsub {xsub}: dL/dB gpu {dB:.6e} oracle {oB:.6e}; dL/dmu gpu {dmu:.6e} oracle {omu:.6e}")
+        assert abs(dB - oB) <= GRAD_RTOL * abs(oB)
+        assert abs(dmu - omu) <= GRAD_RTOL * abs(omu)
+        assert np.linalg.norm(dv0 - odv0) <= GRAD_RTOL * np.linalg.norm(odv0)
+
+
 def test_batched_adjoint_equals_single_scene():
     from simplefluid_amd import engine
     cases = [case(n3=(8, 10, 8), seed=s, B=40.0 + 20 * s, mu=0.01 * (s + 1)) for s in range(3)]
