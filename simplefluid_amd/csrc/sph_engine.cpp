@@ -236,6 +236,7 @@ static void prof_record(sph_handle* h, int slot) {
 
 int enqueue_step(sph_handle* h, StepBufs& w, bool record_prof) {
     hipStream_t st = h->stream;
+    const uint8_t hashed = h->keys_valid ? 0 : 1;
     if (record_prof) prof_record(h, 0);
     if (!h->keys_valid) {
         // New state: counts start from zero, then every particle is hashed.
@@ -305,6 +306,7 @@ int enqueue_step(sph_handle* h, StepBufs& w, bool record_prof) {
     if (record_prof) {
         prof_record(h, 5);
         h->prof_used += 6;
+        h->prof_hashed.push_back(hashed);
     }
     h->has_step = true;
     return SPH_OK;
@@ -839,7 +841,10 @@ int sph_profile_read(sph_handle* h, double* ms, uint32_t* steps) {
     hipError_t e = hipStreamSynchronize(h->stream);
     if (e != hipSuccess) return hip_fail(h, e, "stream synchronize");
     for (size_t b = 0; b + 6 <= h->prof_used; b += 6) {
-        for (int p = 0; p < SPH_NUM_PASSES; ++p) {
+        // A step that launched no hash kernel reports 0 for that pass (its two
+        // events are adjacent; their gap is event overhead, not work).
+        const bool hashed = b / 6 < h->prof_hashed.size() && h->prof_hashed[b / 6];
+        for (int p = hashed ? 0 : 1; p < SPH_NUM_PASSES; ++p) {
             float t = 0.f;
             if (hipEventElapsedTime(&t, h->prof_events[b + p], h->prof_events[b + p + 1]) == hipSuccess)
                 h->prof_ms[p] += t;
@@ -847,6 +852,7 @@ int sph_profile_read(sph_handle* h, double* ms, uint32_t* steps) {
         h->prof_steps++;
     }
     h->prof_used = 0;
+    h->prof_hashed.clear();
     if (ms) for (int p = 0; p < SPH_NUM_PASSES; ++p) ms[p] = h->prof_ms[p];
     if (steps) *steps = h->prof_steps;
     for (int p = 0; p < SPH_NUM_PASSES; ++p) h->prof_ms[p] = 0;

@@ -131,6 +131,7 @@ struct sph_handle {
     // Profiling.
     bool prof = false;
     std::vector<hipEvent_t> prof_events;   // 6 per recorded step
+    std::vector<uint8_t> prof_hashed;      // per recorded step: the hash pass ran
     size_t prof_used = 0;
     double prof_ms[SPH_NUM_PASSES] = {0, 0, 0, 0, 0};
     uint32_t prof_steps = 0;

@@ -257,6 +257,29 @@ def test_graph_replay_equals_eager(E):
     assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
 
 
+def test_profile_counts_the_hash_only_when_it_runs(E):
+    """Per-pass timings: the hash pass runs once after a state upload (later
+    keys come from the force pass), so only that step reports time for it."""
+    from simplefluid_amd import configs
+    cfg = configs.dambreak_65k()
+    e = make_engine(E, cfg["params"], cfg["pos"])
+    e.profile_enable(True)
+    e.advance_steps(1)
+    ms, n = e.profile_read()
+    assert n == 1 and all(v > 0 for v in ms.values()), ms
+    e.advance_steps(5)
+    ms, n = e.profile_read()
+    assert n == 5 and ms["hash"] == 0.0, ms
+    assert all(ms[k] > 0 for k in ("sort", "reorder", "density", "force")), ms
+    e.set_particles(cfg["pos"])
+    e.make_ready()
+    e.advance_steps(2)
+    ms, n = e.profile_read()
+    assert n == 2 and ms["hash"] > 0, ms
+    e.profile_enable(False)
+    e.close()
+
+
 def test_runs_are_deterministic_and_tuning_invariant(E):
     from simplefluid_amd import configs
     cfg = configs.dambreak_65k(jitter=1e-6, seed=4)
