@@ -160,7 +160,18 @@ def host_info():
     except OSError:
         pass
     import glob
-    return {"model": model, "logical_cpus": os.cpu_count(),
+    sockets = set()
+    for f in glob.glob("/sys/devices/system/cpu/cpu[0-9]*/topology/physical_package_id"):
+        try:
+            sockets.add(open(f).read().strip())
+        except OSError:
+            pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count()
+    return {"model": model, "logical_cpus": os.cpu_count(), "usable_cpus": usable,
+            "sockets": len(sockets) or None,
             "numa_nodes": len(glob.glob("/sys/devices/system/node/node[0-9]*"))}
 
 
@@ -205,13 +216,13 @@ def cpu_baseline(cfg, budget_s):
         return rates[1], k
 
     n = cfg["pos"].shape[0]
-    v, k = run(cfg, threads, budget_s, 20, 2)
+    v, k = run(cfg, threads, budget_s, 20, 9)
     v1, k1 = run(cfg, 1, budget_s / 3.0, 3, 1)
     small = configs.dambreak_65k()
     vs, ks = run(small, threads, budget_s / 3.0, 300, 10)
     return {"value": round(v, 1), "unit": "particle-steps/s", "cores": threads, "kind": "port",
             "sample": f"oracle C++ FP32 step (OpenMP, {threads} threads), {n} particles of the same workload: "
-                      f"median of 3 chunks of {k} steps after 3 warm-up steps",
+                      f"median of 3 chunks of {k} steps after 10 warm-up steps",
             "single_thread": {"value": round(v1, 1), "sample": f"median of 3 chunks of {k1} steps"},
             "dambreak_65k": {"value": round(vs, 1), "threads": threads,
                              "sample": f"median of 3 chunks of {ks} of its 1000 steps after 11 warm-up steps"},
