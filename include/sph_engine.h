@@ -163,6 +163,18 @@ int  sph_rollout_forward(sph_handle* h, uint32_t steps, uint32_t checkpoint_ever
 int  sph_rollout_backward(sph_handle* h, const float* dL_dx_final, float* dL_dstiffness,
                           float* dL_dviscosity, float* dL_dv0);
 
+/* ---- device memory (no reference counterpart) ---- */
+/* Caps the device memory this handle's buffers may hold (bytes; 0 = no cap,
+ * the default).  Every call on the handle charges the buffers it allocates to
+ * it; an allocation that would pass the cap is not made, the call returns
+ * SPH_E_OOM with the limit in sph_last_error, and the handle keeps working
+ * once the cap is raised.  The HIP runtime's own allocations (graphs, events,
+ * page-locked download buffers) are not counted.  For callers that share a
+ * GPU between several handles or with other work.                           */
+int  sph_set_device_memory_limit(sph_handle* h, uint64_t bytes);
+/* Device bytes the handle's buffers hold now.                               */
+int  sph_device_memory_used(const sph_handle* h, uint64_t* bytes);
+
 /* ---- measurement ---- */
 #define SPH_PASS_HASH      0   /* standalone cell hash (first step only)        */
 #define SPH_PASS_SORT      1   /* cell-start scan + sort of keys                */

@@ -35,6 +35,7 @@ struct Saved {
 }  // namespace
 
 extern "C" int sph_autotune(sph_handle* h) {
+    BudgetScope budget_scope(h ? &h->mem : nullptr);
     if (!h) return SPH_E_INVALID;
     if (!h->ready) return fail(h, SPH_E_NOT_READY, "call sph_make_ready first");
     const std::string key = cache_key(h);

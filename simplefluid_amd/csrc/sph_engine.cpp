@@ -27,10 +27,19 @@ namespace sph {
 // Message of the last failure on this thread that had no handle to carry it
 // (sph_create, sph_batch_create); sph_last_error(NULL) returns it.
 thread_local std::string t_last_error = "no error";
+thread_local MemBudget* t_budget = nullptr;
 
 int fail(sph_handle* h, int code, const std::string& msg) {
-    if (h) h->last_error = msg;
-    else t_last_error = msg;
+    if (h && code == SPH_E_OOM && h->mem.refused) {
+        h->last_error = msg + " (an allocation of " + std::to_string(h->mem.refused) +
+                        " bytes would exceed the handle's device memory limit of " + std::to_string(h->mem.limit) +
+                        " bytes; " + std::to_string(h->mem.used) + " in use)";
+        h->mem.refused = 0;
+    } else if (h) {
+        h->last_error = msg;
+    } else {
+        t_last_error = msg;
+    }
     return code;
 }
 
@@ -464,12 +473,14 @@ int sph_create(const sph_params* p, sph_handle** out) {
 }
 
 void sph_destroy(sph_handle* h) {
+    BudgetScope budget_scope(h ? &h->mem : nullptr);
     if (!h) return;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     delete h;
 }
 
 int sph_set_params(sph_handle* h, const sph_params* p) {
+    BudgetScope budget_scope(h ? &h->mem : nullptr);
     if (!h || !p) return SPH_E_INVALID;
     if (h->batched) return fail(h, SPH_E_INVALID, "use sph_set_scene_params on a batch handle");
     SceneConst tmp;
@@ -525,6 +536,7 @@ static int upload_state(sph_handle* h, const float* xyz, const float* vel, const
 }
 
 int sph_set_particles(sph_handle* h, const float* xyz, const float* vel, uint32_t n) {
+    BudgetScope budget_scope(h ? &h->mem : nullptr);
     if (!h) return SPH_E_INVALID;
     if (n > 0 && !xyz) return fail(h, SPH_E_INVALID, "xyz is NULL");
     if (h->batched) {
@@ -540,6 +552,7 @@ int sph_set_particles(sph_handle* h, const float* xyz, const float* vel, uint32_
 }
 
 int sph_make_ready(sph_handle* h) {
+    BudgetScope budget_scope(h ? &h->mem : nullptr);
     if (!h) return SPH_E_INVALID;
     if (!h->has_particles) return fail(h, SPH_E_NOT_READY, "no particles set");
     if (h->n == 0) return fail(h, SPH_E_NOT_READY, "zero particles");
@@ -576,6 +589,7 @@ static int capture_graph(sph_handle* h) {
 }
 
 int sph_advance_frame(sph_handle* h, float* dt_out) {
+    BudgetScope budget_scope(h ? &h->mem : nullptr);
     if (!h) return SPH_E_INVALID;
     int rc = sph_advance_steps(h, 1);
     if (rc != SPH_OK) return rc;
@@ -591,6 +605,7 @@ int sph_advance_frame(sph_handle* h, float* dt_out) {
 }
 
 int sph_advance_steps(sph_handle* h, uint32_t k) {
+    BudgetScope budget_scope(h ? &h->mem : nullptr);
     if (!h) return SPH_E_INVALID;
     if (!h->ready) return fail(h, SPH_E_NOT_READY, "call sph_make_ready first");
     for (uint32_t i = 0; i < k; ++i) {
@@ -613,6 +628,7 @@ int sph_advance_steps(sph_handle* h, uint32_t k) {
 }
 
 int sph_run_frame(sph_handle* h, float* frame_time_out, uint32_t* steps_out) {
+    BudgetScope budget_scope(h ? &h->mem : nullptr);
     if (!h) return SPH_E_INVALID;
     float frame_time = 0.0f;
     uint32_t steps = 0;
@@ -680,10 +696,17 @@ static int fetch3(sph_handle* h, const float4* src, float* out) {
     return SPH_OK;
 }
 
-int sph_get_particles(sph_handle* h, float* xyz_out) { return fetch3(h, h ? h->pos.p : nullptr, xyz_out); }
-int sph_get_velocity(sph_handle* h, float* vel_out) { return fetch3(h, h ? h->vel.p : nullptr, vel_out); }
+int sph_get_particles(sph_handle* h, float* xyz_out) {
+    BudgetScope budget_scope(h ? &h->mem : nullptr);
+    return fetch3(h, h ? h->pos.p : nullptr, xyz_out);
+}
+int sph_get_velocity(sph_handle* h, float* vel_out) {
+    BudgetScope budget_scope(h ? &h->mem : nullptr);
+    return fetch3(h, h ? h->vel.p : nullptr, vel_out);
+}
 
 int sph_get_density(sph_handle* h, float* rho_out) {
+    BudgetScope budget_scope(h ? &h->mem : nullptr);
     if (!h || !rho_out) return SPH_E_INVALID;
     if (!h->has_step) return fail(h, SPH_E_NOT_READY, "no step has run");
     // Density is in the last step's sorted order, which is the state order.
@@ -694,6 +717,7 @@ int sph_get_density(sph_handle* h, float* rho_out) {
 }
 
 int sph_snapshot_begin(sph_handle* h) {
+    BudgetScope budget_scope(h ? &h->mem : nullptr);
     if (!h) return SPH_E_INVALID;
     if (!h->has_particles || h->n == 0) return fail(h, SPH_E_NOT_READY, "no particles set");
     const size_t cnt = 3 * (size_t)h->n;
@@ -731,6 +755,7 @@ int sph_snapshot_begin(sph_handle* h) {
 }
 
 int sph_snapshot_wait(sph_handle* h, float* xyz_out) {
+    BudgetScope budget_scope(h ? &h->mem : nullptr);
     if (!h || !xyz_out) return SPH_E_INVALID;
     if (!h->snap_pending) return fail(h, SPH_E_NOT_READY, "no snapshot in flight (call sph_snapshot_begin)");
     const hipError_t e = hipEventSynchronize(h->snap_done);
@@ -741,6 +766,7 @@ int sph_snapshot_wait(sph_handle* h, float* xyz_out) {
 }
 
 int sph_export_frame(sph_handle* h, const char* path) {
+    BudgetScope budget_scope(h ? &h->mem : nullptr);
     if (!h || !path) return SPH_E_INVALID;
     if (!h->has_particles) return fail(h, SPH_E_NOT_READY, "no particles set");
     const size_t n = h->n;
@@ -759,6 +785,7 @@ int sph_export_frame(sph_handle* h, const char* path) {
 }
 
 int sph_synchronize(sph_handle* h) {
+    BudgetScope budget_scope(h ? &h->mem : nullptr);
     if (!h) return SPH_E_INVALID;
     return sync(h);
 }
@@ -766,6 +793,7 @@ int sph_synchronize(sph_handle* h) {
 const char* sph_last_error(const sph_handle* h) { return h ? h->last_error.c_str() : t_last_error.c_str(); }
 
 int sph_set_use_graph(sph_handle* h, int on) {
+    BudgetScope budget_scope(h ? &h->mem : nullptr);
     if (!h) return SPH_E_INVALID;
     h->use_graph = on != 0;
     if (!h->use_graph) invalidate_graph(h);
@@ -784,6 +812,7 @@ int sph_batch_create(const sph_scene_desc* scenes, uint32_t nscenes, sph_handle*
     std::vector<uint32_t> ns(nscenes);
     for (uint32_t s = 0; s < nscenes; ++s) { ps[s] = scenes[s].params; ns[s] = scenes[s].num_particles; }
     h->batched = true;
+    BudgetScope budget_scope(&h->mem);
     rc = configure(h, ps, ns);
     if (rc == SPH_OK) rc = ensure_buffers(h);
     if (rc != SPH_OK) {
@@ -804,6 +833,7 @@ int sph_scene_offsets(const sph_handle* h, uint32_t* offsets) {
 }
 
 int sph_set_scene_params(sph_handle* h, uint32_t scene, const sph_params* p) {
+    BudgetScope budget_scope(h ? &h->mem : nullptr);
     if (!h || !p) return SPH_E_INVALID;
     if (scene >= h->sc_h.size()) return fail(h, SPH_E_INVALID, "scene index out of range");
     const sph_params& o = h->params[scene];
@@ -828,15 +858,31 @@ int sph_set_scene_params(sph_handle* h, uint32_t scene, const sph_params* p) {
     return hip_fail(h, e, "scene table upload");
 }
 
+// ---- device memory ----
+
+int sph_set_device_memory_limit(sph_handle* h, uint64_t bytes) {
+    if (!h) return SPH_E_INVALID;
+    h->mem.limit = bytes;
+    return SPH_OK;
+}
+
+int sph_device_memory_used(const sph_handle* h, uint64_t* bytes) {
+    if (!h || !bytes) return SPH_E_INVALID;
+    *bytes = h->mem.used;
+    return SPH_OK;
+}
+
 // ---- measurement ----
 
 int sph_profile_enable(sph_handle* h, int on) {
+    BudgetScope budget_scope(h ? &h->mem : nullptr);
     if (!h) return SPH_E_INVALID;
     h->prof = on != 0;
     return SPH_OK;
 }
 
 int sph_profile_read(sph_handle* h, double* ms, uint32_t* steps) {
+    BudgetScope budget_scope(h ? &h->mem : nullptr);
     if (!h) return SPH_E_INVALID;
     hipError_t e = hipStreamSynchronize(h->stream);
     if (e != hipSuccess) return hip_fail(h, e, "stream synchronize");
@@ -873,6 +919,7 @@ int sph_get_tuning(const sph_handle* h, int* cfg) {
 static bool valid_block(int b) { return b == 64 || b == 128 || b == 256 || b == 512; }
 
 int sph_set_tuning(sph_handle* h, const int* cfg) {
+    BudgetScope budget_scope(h ? &h->mem : nullptr);
     if (!h || !cfg) return SPH_E_INVALID;
     if (!valid_block(cfg[0]) || !valid_block(cfg[1]) || !valid_block(cfg[2]))
         return fail(h, SPH_E_INVALID, "block sizes must be 64, 128, 256 or 512");
@@ -889,6 +936,7 @@ int sph_set_tuning(sph_handle* h, const int* cfg) {
 }
 
 int sph_set_sort_mode(sph_handle* h, int mode) {
+    BudgetScope budget_scope(h ? &h->mem : nullptr);
     if (!h) return SPH_E_INVALID;
     if (mode != SPH_SORT_CELL_COUNT && mode != SPH_SORT_RADIX) return fail(h, SPH_E_INVALID, "unknown sort mode");
     if (mode == SPH_SORT_RADIX && h->passes > kMaxPasses) return fail(h, SPH_E_INVALID, "key space too large");
@@ -904,6 +952,7 @@ int sph_set_sort_mode(sph_handle* h, int mode) {
 int sph_get_sort_mode(const sph_handle* h) { return h ? h->tune.sort_mode : SPH_E_INVALID; }
 
 int sph_set_cell_subdivision(sph_handle* h, int xsub) {
+    BudgetScope budget_scope(h ? &h->mem : nullptr);
     if (!h) return SPH_E_INVALID;
     if (xsub != 0 && xsub != 1 && xsub != 2 && xsub != 4) return fail(h, SPH_E_INVALID, "x subdivision must be 0, 1, 2 or 4");
     h->xsub_override = xsub;
@@ -944,6 +993,7 @@ int sph_debug_constants(const sph_handle* h, uint32_t scene, float* out14, int* 
 }
 
 int sph_debug_get_state(sph_handle* h, float* xyz, float* vel, uint32_t* ids) {
+    BudgetScope budget_scope(h ? &h->mem : nullptr);
     if (!h) return SPH_E_INVALID;
     if (!h->has_particles) return fail(h, SPH_E_NOT_READY, "no particles set");
     const size_t n = h->n;
@@ -963,6 +1013,7 @@ int sph_debug_get_state(sph_handle* h, float* xyz, float* vel, uint32_t* ids) {
 }
 
 int sph_debug_set_state(sph_handle* h, const float* xyz, const float* vel, const uint32_t* ids) {
+    BudgetScope budget_scope(h ? &h->mem : nullptr);
     if (!h || !xyz) return SPH_E_INVALID;
     if (ids) {  // ids index the original-order outputs: they must be a permutation of 0..n-1
         std::vector<uint8_t> seen(h->n, 0);
@@ -976,6 +1027,7 @@ int sph_debug_set_state(sph_handle* h, const float* xyz, const float* vel, const
 }
 
 int sph_debug_keep_acc(sph_handle* h, int on) {
+    BudgetScope budget_scope(h ? &h->mem : nullptr);
     if (!h) return SPH_E_INVALID;
     h->keep_acc = on != 0;
     invalidate_graph(h);
@@ -987,6 +1039,7 @@ int sph_debug_keep_acc(sph_handle* h, int on) {
 
 int sph_debug_step_outputs(sph_handle* h, uint32_t* keys_in, uint32_t* perm, uint32_t* sorted_keys,
                            float* rho, float* acc3) {
+    BudgetScope budget_scope(h ? &h->mem : nullptr);
     if (!h) return SPH_E_INVALID;
     if (!h->has_step) return fail(h, SPH_E_NOT_READY, "no step has run");
     int rc = sync(h);
@@ -1015,6 +1068,7 @@ int sph_debug_step_outputs(sph_handle* h, uint32_t* keys_in, uint32_t* perm, uin
 uint32_t sph_debug_num_cells(const sph_handle* h) { return h ? h->ncells : 0u; }
 
 int sph_debug_cells(sph_handle* h, uint32_t* start, uint32_t* end) {
+    BudgetScope budget_scope(h ? &h->mem : nullptr);
     if (!h || !start || !end) return SPH_E_INVALID;
     if (!h->has_step) return fail(h, SPH_E_NOT_READY, "no step has run");
     int rc = sync(h);
@@ -1032,6 +1086,7 @@ int sph_debug_cells(sph_handle* h, uint32_t* start, uint32_t* end) {
 }
 
 int sph_debug_neighbors(sph_handle* h, uint32_t* offsets, uint32_t* list, uint64_t capacity, uint64_t* count) {
+    BudgetScope budget_scope(h ? &h->mem : nullptr);
     if (!h || !offsets) return SPH_E_INVALID;
     if (!h->has_step) return fail(h, SPH_E_NOT_READY, "no step has run");
     const size_t n = h->n;

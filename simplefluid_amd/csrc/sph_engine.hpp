@@ -11,24 +11,58 @@
 
 namespace sph {
 
+// Device bytes a handle's buffers hold, against an optional cap.
+struct MemBudget {
+    uint64_t limit = 0;     // 0: no cap
+    uint64_t used = 0;
+    uint64_t refused = 0;   // size of the last allocation the cap refused
+};
+
+// The budget that buffers allocated on this thread are charged to: each C-ABI
+// entry point sets it to its handle's for the duration of the call (a handle
+// is used by one thread at a time).  A buffer remembers the budget it was
+// charged to and credits it back when freed.
+extern thread_local MemBudget* t_budget;
+
+struct BudgetScope {
+    MemBudget* prev;
+    explicit BudgetScope(MemBudget* b) : prev(t_budget) { t_budget = b; }
+    ~BudgetScope() { t_budget = prev; }
+    BudgetScope(const BudgetScope&) = delete;
+    BudgetScope& operator=(const BudgetScope&) = delete;
+};
+
 // Device buffer owned by the handle.
 template <class T>
 struct DevBuf {
     T* p = nullptr;
     size_t count = 0;
+    MemBudget* owner = nullptr;
     ~DevBuf() { release(); }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) {
+            (void)hipFree(p);
+            if (owner) owner->used -= (uint64_t)count * sizeof(T);
+        }
         p = nullptr;
         count = 0;
+        owner = nullptr;
     }
     hipError_t ensure(size_t n) {
         if (n <= count && p) return hipSuccess;
         release();
         if (n == 0) return hipSuccess;
-        hipError_t e = hipMalloc((void**)&p, n * sizeof(T));
+        const uint64_t bytes = (uint64_t)n * sizeof(T);
+        MemBudget* b = t_budget;
+        if (b && b->limit && b->used + bytes > b->limit) {
+            b->refused = bytes;
+            return hipErrorOutOfMemory;
+        }
+        hipError_t e = hipMalloc((void**)&p, bytes);
         if (e != hipSuccess) { p = nullptr; return e; }
         count = n;
+        owner = b;
+        if (b) b->used += bytes;
         return hipSuccess;
     }
 };
@@ -79,6 +113,8 @@ struct RolloutState {
 }  // namespace sph
 
 struct sph_handle {
+    // Declared first so that it outlives every buffer charged to it.
+    sph::MemBudget mem;
     // Configuration.
     std::vector<sph_params> params;        // per scene
     std::vector<uint32_t> scene_n;         // particles per scene

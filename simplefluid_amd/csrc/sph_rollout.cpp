@@ -40,6 +40,7 @@ hipError_t load_checkpoint(sph_handle* h, RolloutState& R, uint32_t slot) {
 extern "C" {
 
 int sph_rollout_forward(sph_handle* h, uint32_t steps, uint32_t every) {
+    BudgetScope budget_scope(h ? &h->mem : nullptr);
     if (!h) return SPH_E_INVALID;
     if (!h->ready) return fail(h, SPH_E_NOT_READY, "call sph_make_ready first");
     if (steps == 0 || every == 0) return fail(h, SPH_E_INVALID, "steps and checkpoint interval must be > 0");
@@ -70,6 +71,7 @@ int sph_rollout_forward(sph_handle* h, uint32_t steps, uint32_t every) {
 }
 
 int sph_rollout_backward(sph_handle* h, const float* dL_dx_final, float* dL_dB, float* dL_dmu, float* dL_dv0) {
+    BudgetScope budget_scope(h ? &h->mem : nullptr);
     if (!h || !dL_dx_final) return SPH_E_INVALID;
     if (!h->rollout || !h->rollout->valid) return fail(h, SPH_E_NOT_READY, "run sph_rollout_forward first");
     RolloutState& R = *h->rollout;

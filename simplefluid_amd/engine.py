@@ -53,7 +53,8 @@ EXPORTS = (
     "sph_get_tuning", "sph_set_tuning", "sph_debug_constants", "sph_debug_get_state",
     "sph_debug_set_state", "sph_debug_keep_acc", "sph_debug_step_outputs",
     "sph_debug_num_cells", "sph_debug_cells", "sph_debug_neighbors", "sph_set_sort_mode",
-    "sph_get_sort_mode", "sph_set_cell_subdivision",
+    "sph_get_sort_mode", "sph_set_cell_subdivision", "sph_set_device_memory_limit",
+    "sph_device_memory_used",
 )
 SORT_CELL_COUNT = 0
 SORT_RADIX = 1
@@ -148,6 +149,8 @@ def load_library(path: str = LIB_PATH):
         "sph_set_sort_mode": (i32, [H, i32]),
         "sph_get_sort_mode": (i32, [H]),
         "sph_set_cell_subdivision": (i32, [H, i32]),
+        "sph_set_device_memory_limit": (i32, [H, C.c_uint64]),
+        "sph_device_memory_used": (i32, [H, P(C.c_uint64)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -328,6 +331,16 @@ class Engine:
 
     def set_use_graph(self, on: bool):
         self._chk(self._L.sph_set_use_graph(self._h, 1 if on else 0))
+
+    # -- device memory ----------------------------------------------------------
+    def set_device_memory_limit(self, nbytes: int):
+        """Cap on the device bytes this handle's buffers may hold (0: none)."""
+        self._chk(self._L.sph_set_device_memory_limit(self._h, int(nbytes)))
+
+    def device_memory_used(self) -> int:
+        v = C.c_uint64(0)
+        self._chk(self._L.sph_device_memory_used(self._h, C.byref(v)))
+        return int(v.value)
 
     # -- measurement ----------------------------------------------------------
     def profile_enable(self, on: bool = True):

@@ -512,6 +512,43 @@ def test_single_particle_and_empty_input(E):
     z.close()
 
 
+def test_device_memory_limit(E):
+    """A handle's buffers are charged to it; an allocation past its cap fails
+    with SPH_E_OOM and a message naming the limit, and the handle works once
+    the cap is raised, bit for bit like an uncapped one."""
+    from simplefluid_amd import configs
+    cfg = configs.dambreak_65k()
+    n = cfg["pos"].shape[0]
+    e = make_engine(E, cfg["params"], cfg["pos"])
+    e.advance_steps(3)
+    used = e.device_memory_used()
+    say(f"device bytes per particle (65k dam break): {used / n:.1f}")
+    assert 100 * n <= used <= 1000 * n
+    e.set_particles(cfg["pos"])
+    e.make_ready()
+    e.advance_steps(3)
+    assert e.device_memory_used() == used, "re-uploading the same count must not grow the buffers"
+    x_ref, v_ref, _ = e.debug_get_state()
+
+    c = E.Engine(cfg["params"])
+    c.set_device_memory_limit(used // 2)
+    with pytest.raises(E.SPHError) as ei:
+        c.set_particles(cfg["pos"])
+        c.make_ready()
+        c.advance_steps(3)
+    assert ei.value.code == E.SPH_E_OOM and "device memory limit" in str(ei.value), str(ei.value)
+    assert c.device_memory_used() <= used // 2
+    c.set_device_memory_limit(0)
+    c.set_particles(cfg["pos"])
+    c.make_ready()
+    c.advance_steps(3)
+    assert c.device_memory_used() == used
+    x, v, _ = c.debug_get_state()
+    assert np.array_equal(x, x_ref) and np.array_equal(v, v_ref)
+    c.close()
+    e.close()
+
+
 def test_error_codes(E):
     from simplefluid_amd import engine
     p = engine.make_params(kernel_radius=-1.0)

@@ -108,6 +108,17 @@ def test_synthetic_configs_shapes():
         assert (s["pos"] < 1.0).all()
 
 
+def test_null_handle_is_rejected(lib):
+    """Entry points given no handle return SPH_E_INVALID without touching a GPU."""
+    from simplefluid_amd import engine
+    v = C.c_uint64(7)
+    assert lib.sph_set_device_memory_limit(None, 1 << 20) == engine.SPH_E_INVALID
+    assert lib.sph_device_memory_used(None, C.byref(v)) == engine.SPH_E_INVALID and v.value == 7
+    assert lib.sph_advance_steps(None, 1) == engine.SPH_E_INVALID
+    assert lib.sph_make_ready(None) == engine.SPH_E_INVALID
+    assert lib.sph_num_particles(None) == 0
+
+
 @pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a GPU driver is present")
 def test_no_gpu_fails_loudly(lib):
     """Without a GPU the engine refuses to run (no CPU fallback) and says why."""
