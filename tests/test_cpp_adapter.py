@@ -10,17 +10,18 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 SRC = os.path.join(ROOT, "tests", "cpp", "qt_adapter_demo.cpp")
+THREADED_SRC = os.path.join(ROOT, "tests", "cpp", "qt_threaded_demo.cpp")
 
 
-def build_demo(out_dir):
+def build_demo(out_dir, src=SRC):
     from simplefluid_amd import engine
     if not os.path.exists(engine.LIB_PATH):
         from simplefluid_amd import build
         build.build()
     libdir = os.path.dirname(engine.LIB_PATH)
-    exe = os.path.join(str(out_dir), "qt_adapter_demo")
-    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", "-Werror", "-I", os.path.join(ROOT, "include"),
-           SRC, "-L", libdir, "-lsph_engine", f"-Wl,-rpath,{libdir}", "-o", exe]
+    exe = os.path.join(str(out_dir), os.path.splitext(os.path.basename(src))[0])
+    cmd = ["g++", "-std=c++17", "-O2", "-pthread", "-Wall", "-Wextra", "-Werror", "-I",
+           os.path.join(ROOT, "include"), src, "-L", libdir, "-lsph_engine", f"-Wl,-rpath,{libdir}", "-o", exe]
     r = subprocess.run(cmd, capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
     return exe
@@ -29,6 +30,7 @@ def build_demo(out_dir):
 def test_adapter_compiles_and_links(tmp_path):
     exe = build_demo(tmp_path)
     assert os.path.exists(exe)
+    assert os.path.exists(build_demo(tmp_path, THREADED_SRC))
 
 
 def test_adapter_header_is_self_contained(tmp_path):
@@ -71,3 +73,18 @@ def test_cpp_driver_matches_python_binding(tmp_path):
     assert np.array_equal(s.getVelocity(), v_cpp)
     lo, hi = np.float32(-1.0) + p.particle_radius, np.float32(1.0) - p.particle_radius
     assert (x_cpp >= lo).all() and (x_cpp <= hi).all()
+
+
+@pytest.mark.gpu
+def test_threaded_driver_snapshots_and_atomic_stop(tmp_path):
+    """Worker thread steps frames and publishes snapshots; the GUI thread reads
+    six of them and stops the worker through a std::atomic flag; the worker's
+    final state equals a single-threaded replay of the same frame count."""
+    from simplefluid_amd import engine as E
+    exe = build_demo(tmp_path, THREADED_SRC)
+    print("[threaded adapter] DoubleDambreak res 16, 6 frames", flush=True)
+    r = subprocess.run([exe, str(E.SCENE_DOUBLE_DAMBREAK), "16", "6"], capture_output=True, text=True,
+                       timeout=150)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "frames read by the GUI thread 6" in r.stdout
