@@ -92,65 +92,90 @@ __global__ void adj_integrate_kernel(AdjArgs a) {
     }
 }
 
+// One particle of the adjoint force pass; returns its (dL/dB, dL/dmu) terms.
+__device__ __forceinline__ double2 adj_force_one(const AdjArgs& a, const SceneConst& sc, uint32_t k) {
+    const float4 pi = a.pos_s[k], vi = a.vel_s[k], abi = a.abar[k];
+    const float Pi = pi.w, qi = vi.w;  // p/rho^2 and 1/rho, stored by the density pass
+    float gx = 0, gy = 0, gz = 0, hx = 0, hy = 0, hz = 0, Pb = 0, qb = 0, Ax = 0, Ay = 0, Az = 0;
+    for_each_neighbor(a, sc, k, pi, [&](uint32_t j) {
+        const float4 pj = a.pos_s[j], vj = a.vel_s[j], abj = a.abar[j];
+        const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+        const float d2 = dist2(rx, ry, rz);
+        const float inv_r = d2 > 0.0f ? __builtin_amdgcn_rsqf(d2) : 0.0f;
+        const float dd = d2 * inv_r;
+        const float hr = sc.h - dd;
+        const float tt = sc.h2 - d2;
+        const float phi = hr * hr * inv_r;
+        const float dphi_r = -tt * inv_r * inv_r * inv_r;
+        const float dax = abi.x - abj.x, day = abi.y - abj.y, daz = abi.z - abj.z;
+        const float dvx = vj.x - vi.x, dvy = vj.y - vi.y, dvz = vj.z - vi.z;
+        const float PP = sc.cP * (Pi + pj.w);
+        const float dar = dax * rx + day * ry + daz * rz;
+        const float dav = dax * dvx + day * dvy + daz * dvz;
+        const float cq = sc.cV * qi * vj.w;
+        const float coefr = PP * dphi_r * dar - cq * inv_r * dav;
+        gx += PP * phi * dax + coefr * rx;
+        gy += PP * phi * day + coefr * ry;
+        gz += PP * phi * daz + coefr * rz;
+        hx -= cq * hr * dax;
+        hy -= cq * hr * day;
+        hz -= cq * hr * daz;
+        Pb += sc.cP * phi * dar;
+        qb += sc.cV * vj.w * hr * dav;
+        Ax += hr * vj.w * dvx;
+        Ay += hr * vj.w * dvy;
+        Az += hr * vj.w * dvz;
+    });
+    // EOS derivatives (clamped Tait, exponent 7).
+    const float rho = a.rho[k];
+    const float x = rho * sc.inv_rho0;
+    const float x2 = x * x, x4 = x2 * x2, x6 = x4 * x2, x7 = x6 * x;
+    const float praw = sc.B * (x7 - 1.0f);
+    const float sgn = praw < 0.0f ? (sc.attractive ? sc.attr : 0.0f) : 1.0f;
+    const float p = praw * sgn;
+    const float dp_drho = sgn * sc.B * 7.0f * x6 * sc.inv_rho0;
+    const float dp_dB = sgn * (x7 - 1.0f);
+    a.rhobar[k] = Pb * (dp_drho * qi * qi - 2.0f * p * qi * qi * qi) - qb * qi * qi;
+    const float bpart = Pb * qi * qi * dp_dB;
+    const float mpart = sc.cP * qi * (abi.x * Ax + abi.y * Ay + abi.z * Az);
+    float4 xs = a.xs[k], vs = a.vs[k];
+    xs.x += gx; xs.y += gy; xs.z += gz;
+    vs.x += hx; vs.y += hy; vs.z += hz;
+    a.xs[k] = xs;
+    a.vs[k] = vs;
+    return make_double2((double)bpart, (double)mpart);
+}
+
 // Eight waves per SIMD (64 VGPRs): 4 % faster than the six its registers
 // would otherwise allow.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) void adj_force_kernel(AdjArgs a) {
     uint32_t t0, t1;
     tile_span(a.ntiles, t0, t1);
+    // Per-tile (dL/dB, dL/dmu) of this step, reduced in double in a fixed
+    // tree and added to the tile's running sum: deterministic, no atomics.
+    __shared__ double2 red[256];
     for (uint32_t t = t0; t < t1; ++t) {
         const uint2 ti = a.tiles[t];
         const SceneConst& sc = a.scenes[ti.x];
         const uint32_t k = ti.y + threadIdx.x;
-        if (k >= sc.p_end) continue;
-        const float4 pi = a.pos_s[k], vi = a.vel_s[k], abi = a.abar[k];
-        const float Pi = pi.w, qi = vi.w;  // p/rho^2 and 1/rho, stored by the density pass
-        float gx = 0, gy = 0, gz = 0, hx = 0, hy = 0, hz = 0, Pb = 0, qb = 0, Ax = 0, Ay = 0, Az = 0;
-        for_each_neighbor(a, sc, k, pi, [&](uint32_t j) {
-            const float4 pj = a.pos_s[j], vj = a.vel_s[j], abj = a.abar[j];
-            const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-            const float d2 = dist2(rx, ry, rz);
-            const float inv_r = d2 > 0.0f ? __builtin_amdgcn_rsqf(d2) : 0.0f;
-            const float dd = d2 * inv_r;
-            const float hr = sc.h - dd;
-            const float tt = sc.h2 - d2;
-            const float phi = hr * hr * inv_r;
-            const float dphi_r = -tt * inv_r * inv_r * inv_r;
-            const float dax = abi.x - abj.x, day = abi.y - abj.y, daz = abi.z - abj.z;
-            const float dvx = vj.x - vi.x, dvy = vj.y - vi.y, dvz = vj.z - vi.z;
-            const float PP = sc.cP * (Pi + pj.w);
-            const float dar = dax * rx + day * ry + daz * rz;
-            const float dav = dax * dvx + day * dvy + daz * dvz;
-            const float cq = sc.cV * qi * vj.w;
-            const float coefr = PP * dphi_r * dar - cq * inv_r * dav;
-            gx += PP * phi * dax + coefr * rx;
-            gy += PP * phi * day + coefr * ry;
-            gz += PP * phi * daz + coefr * rz;
-            hx -= cq * hr * dax;
-            hy -= cq * hr * day;
-            hz -= cq * hr * daz;
-            Pb += sc.cP * phi * dar;
-            qb += sc.cV * vj.w * hr * dav;
-            Ax += hr * vj.w * dvx;
-            Ay += hr * vj.w * dvy;
-            Az += hr * vj.w * dvz;
-        });
-        // EOS derivatives (clamped Tait, exponent 7).
-        const float rho = a.rho[k];
-        const float x = rho * sc.inv_rho0;
-        const float x2 = x * x, x4 = x2 * x2, x6 = x4 * x2, x7 = x6 * x;
-        const float praw = sc.B * (x7 - 1.0f);
-        const float sgn = praw < 0.0f ? (sc.attractive ? sc.attr : 0.0f) : 1.0f;
-        const float p = praw * sgn;
-        const float dp_drho = sgn * sc.B * 7.0f * x6 * sc.inv_rho0;
-        const float dp_dB = sgn * (x7 - 1.0f);
-        a.rhobar[k] = Pb * (dp_drho * qi * qi - 2.0f * p * qi * qi * qi) - qb * qi * qi;
-        a.bpart[k] = Pb * qi * qi * dp_dB;
-        a.mpart[k] = sc.cP * qi * (abi.x * Ax + abi.y * Ay + abi.z * Az);
-        float4 xs = a.xs[k], vs = a.vs[k];
-        xs.x += gx; xs.y += gy; xs.z += gz;
-        vs.x += hx; vs.y += hy; vs.z += hz;
-        a.xs[k] = xs;
-        a.vs[k] = vs;
+        double2 part = make_double2(0.0, 0.0);
+        if (k < sc.p_end) part = adj_force_one(a, sc, k);
+        red[threadIdx.x] = part;
+        __syncthreads();
+        for (uint32_t o = blockDim.x >> 1; o > 0; o >>= 1) {
+            if (threadIdx.x < o) {
+                red[threadIdx.x].x += red[threadIdx.x + o].x;
+                red[threadIdx.x].y += red[threadIdx.x + o].y;
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            double2 acc = a.tile_acc[t];
+            acc.x += red[0].x;
+            acc.y += red[0].y;
+            a.tile_acc[t] = acc;
+        }
+        __syncthreads();
     }
 }
 
@@ -182,25 +207,19 @@ __global__ void adj_density_kernel(AdjArgs a) {
     }
 }
 
-// Per-scene sum of a per-particle array in a fixed order (one block per
-// scene), added to a double accumulator: deterministic run to run.
-// Per-scene sums of two per-particle arrays in double, in a fixed order:
-// block (c, s) sums particles [p_begin + c 4096, + 4096) of scene s (threads
-// stride, then a tree); scene_total_kernel adds a scene's blocks in order.
-constexpr uint32_t kSumChunk = 4096;
-
-__global__ __launch_bounds__(256) void scene_partial_kernel(const SceneConst* __restrict__ scenes,
-                                                            const float* __restrict__ pa,
-                                                            const float* __restrict__ pb,
-                                                            double2* __restrict__ partial, uint32_t chunks) {
+// Per-scene totals of the per-tile accumulators: block s sums tiles
+// [tile_first[s], tile_first[s + 1]) in a fixed order (threads stride, then
+// a tree), so the result is the same run to run.
+__global__ __launch_bounds__(256) void scene_tiles_kernel(const uint32_t* __restrict__ tile_first,
+                                                          const double2* __restrict__ tile_acc,
+                                                          double* __restrict__ acc_a, double* __restrict__ acc_b) {
     __shared__ double sa[256], sb[256];
-    const SceneConst& sc = scenes[blockIdx.y];
-    const uint32_t beg = sc.p_begin + blockIdx.x * kSumChunk;
-    const uint32_t end = beg + kSumChunk < sc.p_end ? beg + kSumChunk : sc.p_end;
+    const uint32_t s = blockIdx.x;
+    const uint32_t beg = tile_first[s], end = tile_first[s + 1];
     double va = 0.0, vb = 0.0;
     for (uint32_t i = beg + threadIdx.x; i < end; i += 256) {
-        va += (double)pa[i];
-        vb += (double)pb[i];
+        va += tile_acc[i].x;
+        vb += tile_acc[i].y;
     }
     sa[threadIdx.x] = va;
     sb[threadIdx.x] = vb;
@@ -212,21 +231,10 @@ __global__ __launch_bounds__(256) void scene_partial_kernel(const SceneConst* __
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) partial[(size_t)blockIdx.y * chunks + blockIdx.x] = make_double2(sa[0], sb[0]);
-}
-
-__global__ void scene_total_kernel(uint32_t nscenes, const double2* __restrict__ partial, uint32_t chunks,
-                                   double* __restrict__ acc_a, double* __restrict__ acc_b) {
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nscenes) return;
-    double a = 0.0, b = 0.0;
-    for (uint32_t c = 0; c < chunks; ++c) {
-        const double2 v = partial[(size_t)s * chunks + c];
-        a += v.x;
-        b += v.y;
+    if (threadIdx.x == 0) {
+        acc_a[s] = sa[0];
+        acc_b[s] = sb[0];
     }
-    acc_a[s] += a;
-    acc_b[s] += b;
 }
 
 __global__ void gather3_kernel(uint32_t n, const float* __restrict__ g, const uint32_t* __restrict__ id,
@@ -247,7 +255,8 @@ hipError_t launch_adj_integrate(hipStream_t st, int block, uint32_t grid, const 
 
 hipError_t launch_adj_force(hipStream_t st, int block, uint32_t grid, const AdjArgs& a) {
     if (grid == 0) return hipSuccess;
-    if (block > 256) return hipErrorInvalidValue;  // __launch_bounds__(256)
+    // __launch_bounds__(256), and the per-tile tree wants a power of two.
+    if (block > 256 || (block & (block - 1))) return hipErrorInvalidValue;
     hipLaunchKernelGGL(adj_force_kernel, dim3(grid), dim3(block), 0, st, a);
     return hipGetLastError();
 }
@@ -258,17 +267,10 @@ hipError_t launch_adj_density(hipStream_t st, int block, uint32_t grid, const Ad
     return hipGetLastError();
 }
 
-uint32_t scene_sum_chunks(uint32_t max_scene_particles) {
-    return (max_scene_particles + kSumChunk - 1) / kSumChunk;
-}
-
-hipError_t launch_scene_sums(hipStream_t st, uint32_t nscenes, uint32_t chunks, const SceneConst* scenes,
-                             const float* pa, const float* pb, double2* partial, double* acc_a, double* acc_b) {
-    if (nscenes == 0 || chunks == 0) return hipSuccess;
-    hipLaunchKernelGGL(scene_partial_kernel, dim3(chunks, nscenes), dim3(256), 0, st, scenes, pa, pb, partial,
-                       chunks);
-    hipLaunchKernelGGL(scene_total_kernel, dim3((nscenes + 63) / 64), dim3(64), 0, st, nscenes, partial, chunks,
-                       acc_a, acc_b);
+hipError_t launch_scene_tile_sums(hipStream_t st, uint32_t nscenes, const uint32_t* tile_first,
+                                  const double2* tile_acc, double* acc_a, double* acc_b) {
+    if (nscenes == 0) return hipSuccess;
+    hipLaunchKernelGGL(scene_tiles_kernel, dim3(nscenes), dim3(256), 0, st, tile_first, tile_acc, acc_a, acc_b);
     return hipGetLastError();
 }
 

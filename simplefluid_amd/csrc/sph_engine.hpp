@@ -104,9 +104,10 @@ struct RolloutState {
     DevBuf<uint32_t> ck_id;
     DevBuf<float4> xbar, vbar;                     // adjoint of the state (state order)
     DevBuf<float4> xs, vs, abar;                   // per-step adjoint temporaries (sorted order)
-    DevBuf<float> rhobar, bpart, mpart;
-    DevBuf<double> gB, gmu;                        // per-scene accumulators
-    DevBuf<double2> gpart;                         // per-scene, per-chunk partial sums
+    DevBuf<float> rhobar;
+    DevBuf<double> gB, gmu;                        // per-scene totals
+    DevBuf<double2> tile_acc;                      // per-tile running (dL/dB, dL/dmu)
+    DevBuf<uint32_t> tile_first;                   // first tile of each scene (nscenes + 1)
     DevBuf<float> gbuf;                            // 3n staging for gradients
 };
 

@@ -118,8 +118,7 @@ struct AdjArgs {
     float4* vs;
     float4* abar;
     float* rhobar;
-    float* bpart;            // per-particle dL/dB term
-    float* mpart;            // per-particle dL/dmu term
+    double2* tile_acc;       // per tile: running (dL/dB, dL/dmu) sums over the reverse steps
     const SceneConst* scenes;
     const uint2* tiles;
     uint32_t ntiles;
@@ -130,11 +129,10 @@ struct AdjArgs {
 hipError_t launch_adj_integrate(hipStream_t st, int block, uint32_t grid, const AdjArgs& a);
 hipError_t launch_adj_force(hipStream_t st, int block, uint32_t grid, const AdjArgs& a);
 hipError_t launch_adj_density(hipStream_t st, int block, uint32_t grid, const AdjArgs& a);
-// acc_a[s] += sum of pa over scene s, acc_b[s] += sum of pb, in double and
-// a fixed order; partial holds nscenes x chunks double2.
-uint32_t scene_sum_chunks(uint32_t max_scene_particles);
-hipError_t launch_scene_sums(hipStream_t st, uint32_t nscenes, uint32_t chunks, const SceneConst* scenes,
-                             const float* pa, const float* pb, double2* partial, double* acc_a, double* acc_b);
+// Per-scene totals of the tile accumulators in a fixed order: scene s sums
+// tiles [tile_first[s], tile_first[s + 1]) into acc_a[s], acc_b[s].
+hipError_t launch_scene_tile_sums(hipStream_t st, uint32_t nscenes, const uint32_t* tile_first,
+                                  const double2* tile_acc, double* acc_a, double* acc_b);
 // out[k] = (g[3 id[k]], g[3 id[k]+1], g[3 id[k]+2], 0)
 hipError_t launch_gather3(hipStream_t st, uint32_t n, const float* g, const uint32_t* id, float4* out);
 

@@ -87,25 +87,29 @@ int sph_rollout_backward(sph_handle* h, const float* dL_dx_final, float* dL_dB, 
     }
     chk(R.xbar.ensure(n)); chk(R.vbar.ensure(n));
     chk(R.xs.ensure(n)); chk(R.vs.ensure(n)); chk(R.abar.ensure(n));
-    chk(R.rhobar.ensure(n)); chk(R.bpart.ensure(n)); chk(R.mpart.ensure(n));
+    chk(R.rhobar.ensure(n));
     chk(R.gB.ensure(ns)); chk(R.gmu.ensure(ns));
-    uint32_t max_scene = 0;
-    for (uint32_t s = 0; s < ns; ++s) max_scene = std::max(max_scene, h->sc_h[s].p_end - h->sc_h[s].p_begin);
-    const uint32_t chunks = scene_sum_chunks(max_scene);
-    chk(R.gpart.ensure((size_t)ns * std::max(chunks, 1u)));
     chk(R.gbuf.ensure(3 * n + 1));
+    uint32_t ntiles = 0;
+    const uint2* tiles = tiles_for(h, 256, ntiles);  // tiles of 256 particles, scene by scene
+    if (!tiles) return fail(h, SPH_E_OOM, "tile table");
+    // First tile of each scene, as the tile table lists them.
+    std::vector<uint32_t> first(ns + 1, 0);
+    for (uint32_t s = 0; s < ns; ++s)
+        first[s + 1] = first[s] + (h->sc_h[s].p_end - h->sc_h[s].p_begin + 255u) / 256u;
+    if (first[ns] != ntiles) return fail(h, SPH_E_INVALID, "tile table does not match the scenes");
+    chk(R.tile_acc.ensure(std::max<uint32_t>(ntiles, 1)));
+    chk(R.tile_first.ensure(ns + 1));
     if (e != hipSuccess) return hip_fail(h, e, "adjoint allocation");
 
     hipStream_t st = h->stream;
-    chk(hipMemsetAsync(R.gB.p, 0, ns * sizeof(double), st));
-    chk(hipMemsetAsync(R.gmu.p, 0, ns * sizeof(double), st));
+    chk(hipMemcpy(R.tile_first.p, first.data(), (ns + 1) * sizeof(uint32_t), hipMemcpyHostToDevice));
+    chk(hipMemsetAsync(R.tile_acc.p, 0, std::max<uint32_t>(ntiles, 1) * sizeof(double2), st));
     chk(hipMemcpyAsync(R.gbuf.p, dL_dx_final, 3 * n * sizeof(float), hipMemcpyHostToDevice, st));
     chk(launch_gather3(st, (uint32_t)n, R.gbuf.p, R.ck_id.p + (size_t)R.nseg * n, R.xbar.p));
     chk(launch_pack3(st, (uint32_t)n, nullptr, R.vbar.p));
     if (e != hipSuccess) return hip_fail(h, e, "adjoint seed");
 
-    uint32_t ntiles = 0;
-    const uint2* tiles = tiles_for(h, 256, ntiles);
     const uint32_t grid = ntiles;  // one block per tile
     for (int seg = (int)R.nseg - 1; seg >= 0; --seg) {
         chk(load_checkpoint(h, R, (uint32_t)seg));
@@ -122,15 +126,15 @@ int sph_rollout_backward(sph_handle* h, const float* dL_dx_final, float* dL_dB, 
             a.pos_s = w.pos_s.p; a.vel_s = w.vel_s.p; a.acc = w.acc.p; a.rho = w.rho.p;
             a.mask = w.mask.p; a.cstart = w.cstart.p; a.perm = w.perm.p;
             a.xbar = R.xbar.p; a.vbar = R.vbar.p; a.xs = R.xs.p; a.vs = R.vs.p; a.abar = R.abar.p;
-            a.rhobar = R.rhobar.p; a.bpart = R.bpart.p; a.mpart = R.mpart.p;
+            a.rhobar = R.rhobar.p; a.tile_acc = R.tile_acc.p;
             a.scenes = h->sc_d.p; a.tiles = tiles; a.ntiles = ntiles; a.n = (uint32_t)n; a.ncells = h->ncells;
             chk(launch_adj_integrate(st, 256, grid, a));
             chk(launch_adj_force(st, 256, grid, a));
-            chk(launch_scene_sums(st, ns, chunks, h->sc_d.p, R.bpart.p, R.mpart.p, R.gpart.p, R.gB.p, R.gmu.p));
             chk(launch_adj_density(st, 256, grid, a));
             if (e != hipSuccess) return hip_fail(h, e, "adjoint launch");
         }
     }
+    chk(launch_scene_tile_sums(st, ns, R.tile_first.p, R.tile_acc.p, R.gB.p, R.gmu.p));
     // vbar is now in the initial state's order (checkpoint 0's ids).
     if (dL_dv0) {
         chk(launch_unpermute3(st, (uint32_t)n, R.vbar.p, R.ck_id.p, R.gbuf.p));
