@@ -234,14 +234,34 @@ def test_double_dambreak_subset_parity(E):
     step_and_compare(E, e, cfg["params"], "double_dambreak slab")
 
 
-def test_reference_scene_defaults_parity(E):
-    """Reference DoubleDambreak at res 24 with the GUI defaults (stiffness
-    50000, viscosity 0.05, box [-1,1]^3)."""
+def test_sphere_drop_subset_parity(E):
+    """The 4.4M sphere drop's physics (r 0.0015, h 0.006, dt 5e-5) on the
+    particles of its lower corner: the sphere's underside above the resting
+    layer on the floor; checked after 20 steps."""
+    from simplefluid_amd import configs
+    cfg = configs.sphere_drop_4m()
+    pos = cfg["pos"]
+    sel = (pos[:, 0] < 0.45) & (pos[:, 1] < 0.35) & (pos[:, 2] < 0.45)
+    sub = pos[sel]
+    say(f"[sphere subset] {len(sub)} of {len(pos)} particles")
+    assert 50_000 < len(sub) < 400_000
+    e = make_engine(E, cfg["params"], sub)
+    e.advance_steps(20)
+    step_and_compare(E, e, cfg["params"], "sphere_drop corner")
+    e.close()
+
+
+@pytest.mark.parametrize("scene", ["SphereDrop", "CubeDrop", "Dambreak", "DoubleDambreak"])
+def test_reference_scene_defaults_parity(E, scene):
+    """Each reference scene (Source/SceneManager.cpp:21-173) at res 24 with
+    the GUI defaults (stiffness 50000, viscosity 0.05, box [-1,1]^3): one step
+    from the same state as the oracle after 25 steps of the engine."""
     p = E.default_params(24)
-    pos = E.scene_generate(E.SCENE_DOUBLE_DAMBREAK, 24)
+    pos = E.scene_generate(E.SCENE_NAMES.index(scene), 24)
     e = make_engine(E, p, pos)
     e.advance_steps(25)
-    step_and_compare(E, e, p, "DoubleDambreak res24")
+    step_and_compare(E, e, p, f"{scene} res24")
+    e.close()
 
 
 def test_graph_replay_equals_eager(E):
