@@ -38,6 +38,9 @@ struct DevBuf {
     T* p = nullptr;
     size_t count = 0;
     MemBudget* owner = nullptr;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;             // owns its allocation
+    DevBuf& operator=(const DevBuf&) = delete;
     ~DevBuf() { release(); }
     void release() {
         if (p) {
