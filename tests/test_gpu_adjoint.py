@@ -122,6 +122,25 @@ def test_adjoint_with_rows_longer_than_the_mask():
         assert np.linalg.norm(dv0 - odv0) <= GRAD_RTOL * np.linalg.norm(odv0)
 
 
+def test_adjoint_attractive_pressure_and_zero_viscosity():
+    """Adjoint through the attractive branch of the EOS (p < 0 scaled by the
+    ratio) and with zero viscosity, against the FP32 reverse-mode oracle."""
+    import oracle
+    from simplefluid_amd import engine
+    for kw in (dict(use_attractive_pressure=1, attractive_pressure_ratio=0.3, viscosity=0.02),
+               dict(viscosity=0.0)):
+        p, pos, vel, target = case(n3=(10, 12, 10), seed=9, B=400.0)
+        for k, v in kw.items():
+            setattr(p, k, v)
+        steps = 12
+        _, oB, omu, odv0 = oracle.rollout_loss_grad(to_oracle(p), pos, vel, target, steps, fp64=False)
+        L, dB, dmu, dv0 = gpu_grads(p, pos, vel, target, steps, 4)[:4]
+        say(f"{kw}: dL/dB gpu {dB:.6e} oracle {oB:.6e}; dL/dmu gpu {dmu:.6e} oracle {omu:.6e}")
+        assert abs(dB - oB) <= GRAD_RTOL * abs(oB)
+        assert abs(dmu - omu) <= GRAD_RTOL * abs(omu)
+        assert np.linalg.norm(dv0 - odv0) <= GRAD_RTOL * np.linalg.norm(odv0)
+
+
 def test_batched_adjoint_equals_single_scene():
     from simplefluid_amd import engine
     cases = [case(n3=(8, 10, 8), seed=s, B=40.0 + 20 * s, mu=0.01 * (s + 1)) for s in range(3)]

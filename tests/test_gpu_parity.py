@@ -234,6 +234,31 @@ def test_double_dambreak_subset_parity(E):
     step_and_compare(E, e, cfg["params"], "double_dambreak slab")
 
 
+PARAM_VARIANTS = {
+    "attractive": dict(pressure_stiffness=5000.0, viscosity=0.01, use_attractive_pressure=1,
+                       attractive_pressure_ratio=0.3),
+    "inviscid_bouncy": dict(pressure_stiffness=500.0, viscosity=0.0, boundary_restitution=0.5,
+                            gravity=(0.0, 0.0, -9.81)),
+    "light_fluid": dict(pressure_stiffness=200.0, viscosity=0.2, rest_density=500.0, dt=2e-4,
+                        gravity=(1.0, -5.0, 0.5)),
+}
+
+
+@pytest.mark.parametrize("variant", sorted(PARAM_VARIANTS))
+def test_parameter_variants_parity(E, variant):
+    """Attractive pressure (p < 0 scaled by the ratio instead of clamped),
+    zero viscosity with a bouncy wall and gravity along z, and another rest
+    density, step, viscosity and gravity: a step from the same state as the
+    oracle after 30 steps of the engine."""
+    from simplefluid_amd import configs
+    cfg = configs.dambreak_65k(jitter=1e-6, seed=31)
+    p = E.make_params(**PARAM_VARIANTS[variant])
+    e = make_engine(E, p, cfg["pos"])
+    e.advance_steps(30)
+    step_and_compare(E, e, p, variant)
+    e.close()
+
+
 def test_sphere_drop_subset_parity(E):
     """The 4.4M sphere drop's physics (r 0.0015, h 0.006, dt 5e-5) on the
     particles of its lower corner: the sphere's underside above the resting
