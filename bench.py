@@ -284,6 +284,8 @@ def small_and_large(args):
     e = engine.Engine(cfg["params"])
     e.set_particles(cfg["pos"])
     e.make_ready()
+    if not args.no_autotune:
+        e.autotune()
     e.advance_steps(5)
     t = time_steps(e, cfg["steps"])
     n = cfg["pos"].shape[0]
@@ -296,6 +298,8 @@ def small_and_large(args):
     e = engine.Engine(cfg["params"])
     e.set_particles(cfg["pos"])
     e.make_ready()
+    if not args.no_autotune:
+        e.autotune()
     e.advance_steps(5)
     k = min(args.sphere_steps, cfg["steps"])
     t = time_steps(e, k)
