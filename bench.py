@@ -290,7 +290,8 @@ def small_and_large(args):
     t = time_steps(e, cfg["steps"])
     n = cfg["pos"].shape[0]
     out["dambreak_65k"] = {"particles": n, "steps": cfg["steps"], "value": round(n * cfg["steps"] / t, 1),
-                           "ms_per_step": round(t / cfg["steps"] * 1e3, 5), "unit": "particle-steps/s"}
+                           "ms_per_step": round(t / cfg["steps"] * 1e3, 5), "unit": "particle-steps/s",
+                           "tuning": e.get_tuning()}
     e.close()
     log("dambreak_65k", out["dambreak_65k"])
     cfg = configs.sphere_drop_4m()
@@ -310,7 +311,7 @@ def small_and_large(args):
     out["sphere_4m"] = {"particles": n, "steps": k, "value": round(n * k / t, 1),
                         "ms_per_step": round(t / k * 1e3, 5), "unit": "particle-steps/s",
                         "pass_ms": {kk: round(v / max(nst, 1), 5) for kk, v in ms.items()},
-                        "x_subdivision": int(e.debug_constants()["xsub"])}
+                        "x_subdivision": int(e.debug_constants()["xsub"]), "tuning": e.get_tuning()}
     e.close()
     log("sphere_4m", out["sphere_4m"])
     return out
