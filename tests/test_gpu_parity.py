@@ -195,6 +195,35 @@ def test_double_dambreak_1m_long_run_stays_physical(E):
     e.close()
 
 
+def test_ensemble_256_full_run_stays_physical(E):
+    """The 256-scene ensemble over its full 1000 steps in one handle (5.7M
+    particles): every scene's particles finite, inside that scene's clamp
+    bounds and falling, ids a permutation of the scene's own range (no
+    particle crosses into another scene)."""
+    from simplefluid_amd import configs
+    ens = configs.ensemble(nscenes=256, seed=3)
+    params = [s["params"] for s in ens["scenes"]]
+    counts = [s["pos"].shape[0] for s in ens["scenes"]]
+    b = E.BatchEngine(params, counts)
+    b.set_particles(np.concatenate([s["pos"] for s in ens["scenes"]]))
+    b.make_ready()
+    say(f"[ensemble run] {sum(counts)} particles, {ens['steps']} steps")
+    b.advance_steps(ens["steps"])
+    x, v = b.get_particles(), b.get_velocity()
+    _, _, ids = b.debug_get_state()
+    off = b.scene_offsets()
+    assert np.isfinite(x).all() and np.isfinite(v).all()
+    for i, s in enumerate(ens["scenes"]):
+        sl = slice(off[i], off[i + 1])
+        p = s["params"]
+        lo = np.array(p.box_min[:], np.float32) + np.float32(p.particle_radius)
+        hi = np.array(p.box_max[:], np.float32) - np.float32(p.particle_radius)
+        assert (x[sl] >= lo).all() and (x[sl] <= hi).all(), i
+        assert np.array_equal(np.sort(ids[sl]), np.arange(off[i], off[i + 1], dtype=np.uint32)), i
+        assert x[sl, 1].mean() < s["pos"][:, 1].mean(), i
+    b.close()
+
+
 def compressed_block_scene():
     """A 2r lattice block beside a block compressed to spacing 1.25r (about
     4x rest density) in the same cell rows: up to 147 neighbours and rows of
@@ -208,6 +237,29 @@ def compressed_block_scene():
     b = np.array([0.1 + 33 * R, R, 0.1 + R]) + s * np.stack([i.ravel(), j.ravel(), k.ravel()], 1)
     pos = np.concatenate([a, b.astype(np.float32)])
     return configs.make_params(pressure_stiffness=50.0, viscosity=0.01, dt=1e-4), pos
+
+
+def test_clustered_cloud_parity(E):
+    """Irregular input: a uniform background, dense Gaussian blobs and
+    particles on two box faces, on two grids; exact keys, cells and
+    neighbour sets, identical density, forces judged against FP64."""
+    rng = np.random.default_rng(23)
+    parts = [rng.uniform(0.0, 1.0, size=(20000, 3))]
+    for c in rng.uniform(0.2, 0.8, size=(6, 3)):
+        parts.append(c + rng.normal(0.0, 0.03, size=(1500, 3)))
+    face = rng.uniform(0.0, 1.0, size=(2000, 3))
+    face[:1000, 0] = 0.0
+    face[1000:, 1] = 1.0
+    parts.append(face)
+    pos = np.clip(np.concatenate(parts), 0.0, 1.0).astype(np.float32)
+    params = E.make_params(pressure_stiffness=50.0, viscosity=0.01, dt=1e-4)
+    for xsub in (1, 4):
+        e = E.Engine(params)
+        e.set_particles(pos)
+        e.set_cell_subdivision(xsub)
+        e.make_ready()
+        step_and_compare(E, e, params, f"clustered cloud xsub {xsub}", vs_fp64=True)
+        e.close()
 
 
 def test_rows_longer_than_the_mask_parity(E):

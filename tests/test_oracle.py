@@ -88,6 +88,37 @@ def test_neighbor_sets_equal_bruteforce(oracle_lib, jitter):
     assert all(k in set(lst[off[k]:off[k + 1]]) for k in range(0, n, 97))
 
 
+@pytest.mark.parametrize("xsub", [1, 4])
+def test_clustered_cloud_neighbors_equal_bruteforce(oracle_lib, xsub):
+    """Random clouds: uniform background, dense Gaussian blobs (cells with tens
+    of particles, rows far longer than a neighbour mask) and particles at the
+    box faces.  Cell-list sets == all-pairs sets."""
+    import oracle
+    rng = np.random.default_rng(17 + xsub)
+    parts = [rng.uniform(0.0, 1.0, size=(1500, 3))]
+    for c in rng.uniform(0.2, 0.8, size=(4, 3)):
+        parts.append(c + rng.normal(0.0, 0.012, size=(500, 3)))
+    face = rng.uniform(0.0, 1.0, size=(200, 3))
+    face[:100, 0] = 0.0
+    face[100:, 1] = 1.0
+    parts.append(face)
+    pos = np.clip(np.concatenate(parts), 0.0, 1.0)
+    p = oracle.make_params(x_subdivision=xsub)
+    o = oracle.Oracle(p)
+    o.record_neighbors(True)
+    o.set_state(pos)
+    o.step()
+    off, lst = o.neighbors()
+    sp = pos[o.step_debug()["perm"]].astype(np.float32)
+    boff, blst = oracle.bruteforce_neighbors(sp, p.kernel_radius)
+    assert np.array_equal(off, boff)
+    counts = np.diff(off)
+    assert counts.max() > 100    # the blobs are dense
+    for k in range(len(pos)):
+        a = lst[off[k]:off[k + 1]]
+        assert np.array_equal(np.sort(a), blst[boff[k]:boff[k + 1]]), k
+
+
 @pytest.mark.parametrize("xsub", [2, 4])
 def test_subdivided_grid_neighbors_and_density(oracle_lib, xsub):
     """x cells of cs/xsub: same neighbour sets as brute force, and density
