@@ -120,7 +120,9 @@ int  sph_synchronize(sph_handle* h);
  * (original order) in stream order and returns at once; steps enqueued after
  * it run while a second stream copies the capture to page-locked memory;
  * wait() blocks until that copy is done and writes 3n floats.  One snapshot
- * is in flight at a time; a begin() waits on the GPU for the previous copy. */
+ * is in flight at a time; a begin() waits on the GPU for the previous copy.
+ * If the particle count changed in between (sph_set_particles), wait()
+ * returns SPH_E_INVALID and drops the snapshot.                             */
 int  sph_snapshot_begin(sph_handle* h);
 int  sph_snapshot_wait(sph_handle* h, float* xyz_out);
 /* State dump of one frame (no reference counterpart: the reference exports

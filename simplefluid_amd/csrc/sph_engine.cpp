@@ -751,6 +751,7 @@ int sph_snapshot_begin(sph_handle* h) {
     if (e == hipSuccess) e = hipEventRecord(h->snap_done, h->copy_stream);
     if (e != hipSuccess) return hip_fail(h, e, "snapshot enqueue");
     h->snap_pending = true;
+    h->snap_count = h->n;
     return SPH_OK;
 }
 
@@ -761,7 +762,11 @@ int sph_snapshot_wait(sph_handle* h, float* xyz_out) {
     const hipError_t e = hipEventSynchronize(h->snap_done);
     if (e != hipSuccess) return hip_fail(h, e, "snapshot wait");
     h->snap_pending = false;
-    std::memcpy(xyz_out, h->snap_host, 3 * (size_t)h->n * sizeof(float));
+    if (h->snap_count != h->n)
+        return fail(h, SPH_E_INVALID, "particle count changed since sph_snapshot_begin (" +
+                                          std::to_string(h->snap_count) + " then, " + std::to_string(h->n) +
+                                          " now); the snapshot is dropped");
+    std::memcpy(xyz_out, h->snap_host, 3 * (size_t)h->snap_count * sizeof(float));
     return check_device_error(h);
 }
 

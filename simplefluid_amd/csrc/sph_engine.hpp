@@ -160,7 +160,8 @@ struct sph_handle {
     hipEvent_t snap_ready = nullptr, snap_done = nullptr;
     sph::DevBuf<float> snap_d;
     float* snap_host = nullptr;
-    size_t snap_n = 0;
+    size_t snap_n = 0;                     // floats snap_d / snap_host hold
+    uint32_t snap_count = 0;               // particles when the pending snapshot began
     bool snap_pending = false;
 
     // Graph of one step.

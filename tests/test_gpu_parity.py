@@ -512,6 +512,22 @@ def test_snapshot_overlaps_later_steps(E):
     assert np.array_equal(e.snapshot_wait(), x2)
     with pytest.raises(E.SPHError):
         e.snapshot_wait()                  # nothing in flight
+    # A new particle set between begin and wait: the snapshot is refused and
+    # dropped, and the next one has the new count.
+    big = np.concatenate([cfg["pos"], cfg["pos"][:1000] + np.float32(0.4)])
+    e.snapshot_begin()
+    e.set_particles(big)
+    e.make_ready()
+    with pytest.raises(E.SPHError) as ei:
+        e.snapshot_wait()
+    assert ei.value.code == E.SPH_E_INVALID and "count changed" in str(ei.value)
+    with pytest.raises(E.SPHError) as ei:
+        e.snapshot_wait()
+    assert ei.value.code == E.SPH_E_NOT_READY
+    e.advance_steps(2)
+    e.snapshot_begin()
+    s2 = e.snapshot_wait()
+    assert s2.shape == big.shape and np.array_equal(s2, e.get_particles())
     e.close()
 
 
