@@ -367,6 +367,7 @@ static int pick_xsub(const std::vector<sph_params>& ps, uint64_t n, int override
 }
 
 static int configure(sph_handle* h, const std::vector<sph_params>& ps, const std::vector<uint32_t>& ns) {
+    ++h->config_gen;
     std::vector<SceneConst> sc(ps.size());
     uint64_t cell_off = 0, p_off = 0;
     std::vector<uint32_t> off(ps.size() + 1, 0);
@@ -504,6 +505,7 @@ int sph_get_params(const sph_handle* h, sph_params* out) {
 }
 
 static int upload_state(sph_handle* h, const float* xyz, const float* vel, const uint32_t* ids) {
+    ++h->config_gen;
     const size_t n = h->n;
     int rc = ensure_buffers(h);
     if (rc != SPH_OK) return rc;
@@ -841,6 +843,7 @@ int sph_set_scene_params(sph_handle* h, uint32_t scene, const sph_params* p) {
     BudgetScope budget_scope(h ? &h->mem : nullptr);
     if (!h || !p) return SPH_E_INVALID;
     if (scene >= h->sc_h.size()) return fail(h, SPH_E_INVALID, "scene index out of range");
+    ++h->config_gen;
     const sph_params& o = h->params[scene];
     const bool same_geom = p->particle_radius == o.particle_radius && p->kernel_radius == o.kernel_radius &&
                            p->dt == o.dt && p->rest_density == o.rest_density &&

@@ -102,6 +102,8 @@ struct RolloutState {
     uint32_t every = 1;
     uint32_t nseg = 0;
     bool valid = false;
+    uint32_t n = 0;                                // particles of the rollout
+    uint64_t gen = 0;                              // handle configuration it was run on
     std::vector<std::unique_ptr<StepBufs>> tape;   // `every` slots
     DevBuf<float4> ck_pos, ck_vel;                 // (nseg + 1) x n; slot nseg = final state
     DevBuf<uint32_t> ck_id;
@@ -177,6 +179,9 @@ struct sph_handle {
     double prof_ms[SPH_NUM_PASSES] = {0, 0, 0, 0, 0};
     uint32_t prof_steps = 0;
 
+    // Bumped by every change of particles, parameters or grid: a rollout
+    // recorded under another configuration cannot be reversed.
+    uint64_t config_gen = 0;
     std::unique_ptr<sph::RolloutState> rollout;
     std::string last_error;
 

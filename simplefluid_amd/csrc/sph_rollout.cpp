@@ -67,6 +67,8 @@ int sph_rollout_forward(sph_handle* h, uint32_t steps, uint32_t every) {
     e = save_checkpoint(h, R, R.nseg);
     if (e != hipSuccess) return hip_fail(h, e, "checkpoint copy");
     R.valid = true;
+    R.n = h->n;
+    R.gen = h->config_gen;
     return SPH_OK;
 }
 
@@ -75,6 +77,9 @@ int sph_rollout_backward(sph_handle* h, const float* dL_dx_final, float* dL_dB, 
     if (!h || !dL_dx_final) return SPH_E_INVALID;
     if (!h->rollout || !h->rollout->valid) return fail(h, SPH_E_NOT_READY, "run sph_rollout_forward first");
     RolloutState& R = *h->rollout;
+    if (R.n != h->n || R.gen != h->config_gen)
+        return fail(h, SPH_E_NOT_READY,
+                    "particles, parameters or grid changed since sph_rollout_forward; run it again");
     const size_t n = h->n;
     const uint32_t ns = (uint32_t)h->sc_h.size();
     hipError_t e = hipSuccess;

@@ -141,6 +141,37 @@ def test_adjoint_attractive_pressure_and_zero_viscosity():
         assert np.linalg.norm(dv0 - odv0) <= GRAD_RTOL * np.linalg.norm(odv0)
 
 
+def test_backward_refuses_a_stale_rollout():
+    """After new particles or new parameters the recorded rollout no longer
+    describes the handle: the reverse pass refuses it (before, a changed
+    particle count made it read past its checkpoints), and a new forward
+    rollout can be reversed again."""
+    from simplefluid_amd import engine
+    p, pos, vel, target = case(n3=(8, 8, 8), seed=2)
+    e = engine.Engine(p)
+    e.set_particles(pos, vel)
+    e.make_ready()
+    g = np.zeros((len(pos), 3), np.float32)
+    e.rollout_forward(6, 3)
+    bigger = np.concatenate([pos, pos[:100] + np.float32(0.3)])
+    e.set_particles(bigger)
+    e.make_ready()
+    with pytest.raises(engine.SPHError) as ei:
+        e.rollout_backward(np.zeros((len(bigger), 3), np.float32))
+    assert ei.value.code == engine.SPH_E_NOT_READY and "changed" in str(ei.value)
+    e.set_particles(pos, vel)
+    e.make_ready()
+    e.rollout_forward(6, 3)
+    p2 = engine.make_params(pressure_stiffness=80.0, viscosity=0.01, dt=1e-4)
+    e.set_params(p2)
+    e.make_ready()
+    with pytest.raises(engine.SPHError):
+        e.rollout_backward(g)
+    e.rollout_forward(6, 3)
+    e.rollout_backward(g)
+    e.close()
+
+
 def test_batched_adjoint_equals_single_scene():
     from simplefluid_amd import engine
     cases = [case(n3=(8, 10, 8), seed=s, B=40.0 + 20 * s, mu=0.01 * (s + 1)) for s in range(3)]
