@@ -42,6 +42,34 @@ def test_adapter_header_is_self_contained(tmp_path):
     assert r.returncode == 0, r.stderr
 
 
+@pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a GPU driver is present")
+def test_adapter_reports_engine_errors_as_exceptions(tmp_path):
+    """Without a GPU, constructing the solver throws sph::Error carrying the
+    engine's code and message (the reference has no error channel)."""
+    from simplefluid_amd import engine
+    if not os.path.exists(engine.LIB_PATH):
+        from simplefluid_amd import build
+        build.build()
+    libdir = os.path.dirname(engine.LIB_PATH)
+    src = tmp_path / "err.cpp"
+    src.write_text(
+        '#include <cstdio>\n#include <memory>\n#include "sph_solver.hpp"\n'
+        "int main() {\n"
+        "  auto p = std::make_shared<sph::SPHParameters>();\n"
+        "  try { sph::QtSPHSolver<> s(p); return 1; }\n"
+        "  catch (const sph::Error& e) { std::printf(\"%d %s\\n\", e.code(), e.what());\n"
+        "    return e.code() == SPH_E_HIP ? 0 : 2; }\n"
+        "}\n")
+    exe = str(tmp_path / "err")
+    r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), str(src),
+                        "-L", libdir, "-lsph_engine", f"-Wl,-rpath,{libdir}", "-o", exe],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "HIP device" in r.stdout
+
+
 @pytest.mark.gpu
 def test_cpp_driver_matches_python_binding(tmp_path):
     from simplefluid_amd import engine as E
