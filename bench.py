@@ -276,10 +276,9 @@ def batched(args):
             "unit": "particle-steps/s"}
 
 
-def small_and_large(args):
-    """The 65k dam break over its full 1000 steps, and the 4M sphere drop."""
+def dambreak_65k_leg(args):
+    """The 65,536-particle dam break over its full 1000 steps."""
     from simplefluid_amd import configs, engine
-    out = {}
     cfg = configs.dambreak_65k()
     e = engine.Engine(cfg["params"])
     e.set_particles(cfg["pos"])
@@ -289,11 +288,17 @@ def small_and_large(args):
     e.advance_steps(5)
     t = time_steps(e, cfg["steps"])
     n = cfg["pos"].shape[0]
-    out["dambreak_65k"] = {"particles": n, "steps": cfg["steps"], "value": round(n * cfg["steps"] / t, 1),
-                           "ms_per_step": round(t / cfg["steps"] * 1e3, 5), "unit": "particle-steps/s",
-                           "tuning": e.get_tuning()}
+    out = {"particles": n, "steps": cfg["steps"], "value": round(n * cfg["steps"] / t, 1),
+           "ms_per_step": round(t / cfg["steps"] * 1e3, 5), "unit": "particle-steps/s",
+           "tuning": e.get_tuning()}
     e.close()
-    log("dambreak_65k", out["dambreak_65k"])
+    log("dambreak_65k", out)
+    return out
+
+
+def sphere_4m_leg(args):
+    """The 4.4M-particle sphere drop over a resting layer."""
+    from simplefluid_amd import configs, engine
     cfg = configs.sphere_drop_4m()
     n = cfg["pos"].shape[0]
     e = engine.Engine(cfg["params"])
@@ -308,13 +313,18 @@ def small_and_large(args):
     e.advance_steps(20)
     ms, nst = e.profile_read()
     e.profile_enable(False)
-    out["sphere_4m"] = {"particles": n, "steps": k, "value": round(n * k / t, 1),
-                        "ms_per_step": round(t / k * 1e3, 5), "unit": "particle-steps/s",
-                        "pass_ms": {kk: round(v / max(nst, 1), 5) for kk, v in ms.items()},
-                        "x_subdivision": int(e.debug_constants()["xsub"]), "tuning": e.get_tuning()}
+    out = {"particles": n, "steps": k, "value": round(n * k / t, 1),
+           "ms_per_step": round(t / k * 1e3, 5), "unit": "particle-steps/s",
+           "pass_ms": {kk: round(v / max(nst, 1), 5) for kk, v in ms.items()},
+           "x_subdivision": int(e.debug_constants()["xsub"]), "tuning": e.get_tuning()}
     e.close()
-    log("sphere_4m", out["sphere_4m"])
+    log("sphere_4m", out)
     return out
+
+
+def small_and_large(args):
+    """The 65k dam break over its full 1000 steps, and the 4.4M sphere drop."""
+    return {"dambreak_65k": dambreak_65k_leg(args), "sphere_4m": sphere_4m_leg(args)}
 
 
 def rollout_targets(rc):
@@ -404,7 +414,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
-    ap.add_argument("--only", choices=["plain", "batched", "adjoint"], default=None,
+    ap.add_argument("--only", choices=["plain", "batched", "adjoint", "dambreak65k", "sphere4m"], default=None,
                     help="run one leg alone (profiling)")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tuning", type=str, default="",
@@ -426,6 +436,12 @@ def main():
         return
     if args.only == "adjoint":
         print(json.dumps({"adjoint": adjoint(args)}), flush=True)
+        return
+    if args.only == "dambreak65k":
+        print(json.dumps({"dambreak_65k": dambreak_65k_leg(args)}), flush=True)
+        return
+    if args.only == "sphere4m":
+        print(json.dumps({"sphere_4m": sphere_4m_leg(args)}), flush=True)
         return
     value, step_ms, roof, extra, cfg = plain(args)
     out = {"metric": METRIC, "value": round(value, 1), "unit": "particle-steps/s", "n_gpus": 1,
