@@ -105,7 +105,10 @@ int  sph_advance_frame(sph_handle* h, float* dt_out);
 /* k steps, replayed from a captured HIP graph when graphs are enabled.     */
 int  sph_advance_steps(sph_handle* h, uint32_t k);
 /* Steps until the accumulated float frame time reaches 0.0333333333
- * (Source/Simulator.cpp:46-51); returns that time and the step count.      */
+ * (Source/Simulator.cpp:46-51); returns that time and the step count.
+ * With cfl_number > 0 the frame runs on the device: a step graph that stops
+ * itself at the end of the frame is replayed in batches, and the host reads
+ * the frame state once per batch rather than every step's length.        */
 int  sph_run_frame(sph_handle* h, float* frame_time_out, uint32_t* steps_out);
 
 uint32_t sph_num_particles(const sph_handle* h);

@@ -120,8 +120,12 @@ int derive_scene(const sph_params& p, SceneConst& sc, std::string& err, int xsub
 void invalidate_graph(sph_handle* h) {
     if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
     if (h->graph) (void)hipGraphDestroy(h->graph);
+    if (h->gexec_frame) (void)hipGraphExecDestroy(h->gexec_frame);
+    if (h->graph_frame) (void)hipGraphDestroy(h->graph_frame);
     h->gexec = nullptr;
     h->graph = nullptr;
+    h->gexec_frame = nullptr;
+    h->graph_frame = nullptr;
 }
 
 uint2* tiles_for(sph_handle* h, int block, uint32_t& ntiles) {
@@ -220,6 +224,7 @@ static int ensure_buffers(sph_handle* h) {
     chk(h->queue.ensure(kQueueWords));
     chk(h->vmax2.ensure(h->sc_h.size()));
     chk(h->dt_dev.ensure(1));
+    chk(h->frame.ensure(kFrameWords));
     chk(h->staging.ensure(3 * n + 1));
     chk(h->sc_d.ensure(h->sc_h.size()));
     if (e != hipSuccess) return hip_fail(h, e, "device allocation");
@@ -243,8 +248,14 @@ static void prof_record(sph_handle* h, int slot) {
     (void)hipEventRecord(h->prof_events[idx], h->stream);
 }
 
-int enqueue_step(sph_handle* h, StepBufs& w, bool record_prof) {
+int enqueue_step(sph_handle* h, StepBufs& w, bool record_prof, uint32_t* frame) {
     hipStream_t st = h->stream;
+    if (frame) {
+        if (!h->keys_valid || h->tune.sort_mode != kSortCellCount)
+            return fail(h, SPH_E_INVALID, "the device frame loop needs a steady cell-count step");
+        const hipError_t ge = launch_frame_gate(st, frame);
+        if (ge != hipSuccess) return hip_fail(h, ge, "frame gate launch");
+    }
     const uint8_t hashed = h->keys_valid ? 0 : 1;
     if (record_prof) prof_record(h, 0);
     if (!h->keys_valid) {
@@ -285,6 +296,7 @@ int enqueue_step(sph_handle* h, StepBufs& w, bool record_prof) {
     sp.id_in = h->id.p;
     sp.pos_out = w.pos_s.p;
     sp.soa_out = w.soa.p;
+    sp.halt = frame;
     sp.vel_out = w.vel_s.p;
     sp.id_out = w.id_s.p;
     hipError_t e = launch_sort_keys(st, sp);
@@ -295,6 +307,7 @@ int enqueue_step(sph_handle* h, StepBufs& w, bool record_prof) {
     if (record_prof) prof_record(h, 3);
     {
         StepArgs a = step_args(h, w, h->tune.density_block);
+        a.halt = frame;
         const uint32_t grid = grid_for(h, h->tune.density_block, a.ntiles, &a.queue);
         e = launch_density(st, h->tune.density_block, grid, a);
         if (e != hipSuccess) return hip_fail(h, e, "density launch");
@@ -302,12 +315,13 @@ int enqueue_step(sph_handle* h, StepBufs& w, bool record_prof) {
     if (record_prof) prof_record(h, 4);
     if (h->cfl) {
         // Step length from the largest speed of the state this step starts from.
-        e = launch_cfl(st, (uint32_t)h->sc_h.size(), h->sc_d.p, h->vmax2.p, h->dt_dev.p);
+        e = launch_cfl(st, (uint32_t)h->sc_h.size(), h->sc_d.p, h->vmax2.p, h->dt_dev.p, frame);
         if (e != hipSuccess) return hip_fail(h, e, "cfl launch");
     }
     {
         StepArgs a = step_args(h, w, h->tune.force_block);
         a.vmax2 = h->cfl ? h->vmax2.p : nullptr;
+        a.halt = frame;
         const uint32_t grid = grid_for(h, h->tune.force_block, a.ntiles, &a.queue);
         e = launch_force(st, h->tune.force_block, grid, a);
         if (e != hipSuccess) return hip_fail(h, e, "force launch");
@@ -569,11 +583,17 @@ static int step_once(sph_handle* h) {
     return enqueue_step(h, h->work, h->prof);
 }
 
-static int capture_graph(sph_handle* h) {
-    invalidate_graph(h);
+// Captures one steady step; gated: the device frame loop's step.
+static int capture_graph(sph_handle* h, bool gated = false) {
+    hipGraphExec_t& gx = gated ? h->gexec_frame : h->gexec;
+    hipGraph_t& gr = gated ? h->graph_frame : h->graph;
+    if (gx) (void)hipGraphExecDestroy(gx);
+    if (gr) (void)hipGraphDestroy(gr);
+    gx = nullptr;
+    gr = nullptr;
     hipError_t e = hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return hip_fail(h, e, "begin capture");
-    const int rc = enqueue_step(h, h->work, false);
+    const int rc = enqueue_step(h, h->work, false, gated ? h->frame.p : nullptr);
     hipGraph_t g = nullptr;
     e = hipStreamEndCapture(h->stream, &g);
     if (rc != SPH_OK) {
@@ -581,8 +601,8 @@ static int capture_graph(sph_handle* h) {
         return rc;
     }
     if (e != hipSuccess) return hip_fail(h, e, "end capture");
-    h->graph = g;
-    e = hipGraphInstantiate(&h->gexec, g, nullptr, nullptr, 0);
+    gr = g;
+    e = hipGraphInstantiate(&gx, g, nullptr, nullptr, 0);
     if (e != hipSuccess) {
         invalidate_graph(h);
         return hip_fail(h, e, "graph instantiate");
@@ -629,11 +649,59 @@ int sph_advance_steps(sph_handle* h, uint32_t k) {
     return SPH_OK;
 }
 
+// Frame loop of the adaptive step on the device: the gated step graph is
+// replayed in batches (sized from the last frame) and the frame state read
+// once per batch, instead of the step length after every step.  Steps
+// launched after the frame is complete do nothing.  The frame time is the
+// same float sum, compared with the same bound, as the host loop's.
+static int run_frame_on_device(sph_handle* h, float& frame_time, uint32_t& steps) {
+    hipStream_t st = h->stream;
+    hipError_t e = launch_frame_reset(st, h->frame.p);
+    if (e != hipSuccess) return hip_fail(h, e, "frame reset");
+    if (!h->gexec_frame) {
+        const int rc = capture_graph(h, true);
+        if (rc != SPH_OK) return rc;
+    }
+    uint32_t batch = h->frame_steps_hint ? h->frame_steps_hint + 1 : 16;
+    uint64_t launched = 0;
+    uint32_t fs[kFrameWords] = {0u, 0u, 0u, 0u};
+    for (;;) {
+        for (uint32_t i = 0; i < batch; ++i) {
+            e = hipGraphLaunch(h->gexec_frame, st);
+            if (e != hipSuccess) return hip_fail(h, e, "graph launch");
+        }
+        launched += batch;
+        e = hipMemcpyAsync(fs, h->frame.p, sizeof(fs), hipMemcpyDeviceToHost, st);
+        if (e != hipSuccess) return hip_fail(h, e, "frame state read");
+        const int rc = sync(h);
+        if (rc != SPH_OK) return rc;
+        if (fs[1]) break;  // the frame reached 1/30 s
+        float ft;
+        std::memcpy(&ft, &fs[3], sizeof(ft));
+        const double per = fs[2] ? (double)ft / (double)fs[2] : 1e-4;
+        const double need = (0.0333333333 - (double)ft) / std::max(per, 1e-12);
+        batch = (uint32_t)std::min(std::max(need + 1.0, 1.0), 65536.0);
+        if (launched > (1ull << 24)) return fail(h, SPH_E_DEVICE, "the frame did not reach 1/30 s");
+    }
+    std::memcpy(&frame_time, &fs[3], sizeof(frame_time));
+    steps = fs[2];
+    h->frame_steps_hint = steps;
+    return SPH_OK;
+}
+
 int sph_run_frame(sph_handle* h, float* frame_time_out, uint32_t* steps_out) {
     BudgetScope budget_scope(h ? &h->mem : nullptr);
     if (!h) return SPH_E_INVALID;
     float frame_time = 0.0f;
     uint32_t steps = 0;
+    if (h->cfl && h->ready && h->use_graph && !h->prof && h->keys_valid && !h->keep_acc &&
+        h->tune.sort_mode == kSortCellCount) {
+        const int rc = run_frame_on_device(h, frame_time, steps);
+        if (rc != SPH_OK) return rc;
+        if (frame_time_out) *frame_time_out = frame_time;
+        if (steps_out) *steps_out = steps;
+        return SPH_OK;
+    }
     while (frame_time < 0.0333333333) {
         float dt = 0.0f;
         int rc = sph_advance_frame(h, &dt);

@@ -170,6 +170,12 @@ struct sph_handle {
     bool use_graph = true;
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
+    // Device frame loop (CFL): a step graph whose kernels skip once the
+    // frame state says the frame is complete; frame holds that state.
+    hipGraphExec_t gexec_frame = nullptr;
+    hipGraph_t graph_frame = nullptr;
+    sph::DevBuf<uint32_t> frame;
+    uint32_t frame_steps_hint = 0;         // steps of the last device-loop frame
 
     // Profiling.
     bool prof = false;
@@ -201,7 +207,9 @@ uint2* tiles_for(sph_handle* h, int block, uint32_t& ntiles);
 uint32_t grid_for(sph_handle* h, int block, uint32_t ntiles, uint32_t** queue);
 StepArgs step_args(sph_handle* h, StepBufs& w, int block);
 // Enqueue one step reading/writing the state buffers with intermediates in w.
-int enqueue_step(sph_handle* h, StepBufs& w, bool record_prof);
+// frame: device frame state (every kernel of the step skips once it is
+// halted; cell-count sort only), or null.
+int enqueue_step(sph_handle* h, StepBufs& w, bool record_prof, uint32_t* frame = nullptr);
 int check_device_error(sph_handle* h);
 int sync(sph_handle* h);
 

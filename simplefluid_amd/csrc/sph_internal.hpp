@@ -34,6 +34,7 @@ struct SortPlan {
     uint32_t* tickets = nullptr;      // radix: kMaxPasses
     uint32_t* err = nullptr;
     uint32_t* queue = nullptr;        // kQueueWords tile counters, zeroed by the scan
+    const uint32_t* halt = nullptr;   // device frame loop: non-zero word skips the step (cell-count mode)
     const float4* pos_in = nullptr;
     const float4* vel_in = nullptr;
     const uint32_t* id_in = nullptr;
@@ -77,6 +78,7 @@ struct StepArgs {
     uint32_t ncells;
     uint32_t* queue;         // persistent launch: 8 tile counters (zeroed by the scan), else null
     uint32_t* vmax2;         // CFL: per-scene max |v|^2 (float bits) of the new state, else null
+    const uint32_t* halt;    // device frame loop: non-zero word skips the step, else null
 };
 
 // Tile counters of the persistent density and force launches (8 each),
@@ -87,8 +89,18 @@ hipError_t launch_hash(hipStream_t st, int block, uint32_t grid, const float4* p
                        uint32_t* key0, uint32_t* rank0, uint32_t* count, uint32_t ncells, uint32_t* vmax2,
                        const SceneConst* scenes, const uint2* tiles, uint32_t ntiles);
 // CFL step of every scene from its max |v|^2 (scenes with cfl = 0 keep dt);
-// clears vmax2 and stores scene 0's step in *dt_out.
-hipError_t launch_cfl(hipStream_t st, uint32_t nscenes, SceneConst* scenes, uint32_t* vmax2, float* dt_out);
+// clears vmax2 and stores scene 0's step in *dt_out.  With a frame state
+// (below) the kernel does nothing once the frame is halted, and otherwise
+// adds scene 0's step to the frame time.
+hipError_t launch_cfl(hipStream_t st, uint32_t nscenes, SceneConst* scenes, uint32_t* vmax2, float* dt_out,
+                      uint32_t* frame);
+// Device frame loop state, 4 words: halt (steps are skipped), pending (the
+// frame reached 1/30 s in the current step), steps taken, frame time (float
+// bits).  reset zeroes them; gate, run first in each step, turns pending
+// into halt, so the step that completes the frame runs to its end.
+constexpr int kFrameWords = 4;
+hipError_t launch_frame_reset(hipStream_t st, uint32_t* frame);
+hipError_t launch_frame_gate(hipStream_t st, uint32_t* frame);
 hipError_t launch_density(hipStream_t st, int block, uint32_t grid, const StepArgs& a);
 hipError_t launch_force(hipStream_t st, int block, uint32_t grid, const StepArgs& a);
 hipError_t launch_fill_u32(hipStream_t st, uint32_t* p, uint32_t value, size_t count);

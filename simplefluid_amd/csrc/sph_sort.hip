@@ -101,7 +101,8 @@ constexpr int kTileScanBlock = 1024;
 constexpr uint32_t kInlineTiles = 1024;
 
 __global__ __launch_bounds__(kTileScanBlock) void tile_scan_kernel(uint32_t* __restrict__ count, uint32_t C,
-                                                                   uint32_t T) {
+                                                                   uint32_t T, const uint32_t* __restrict__ halt) {
+    if (halt && *halt) return;
     // Rounds of 4096 tile sums: thread t takes 4 consecutive ones (16-byte
     // access when aligned), a block scan adds the round's carry.
     __shared__ uint32_t s_w[kTileScanBlock / 64];
@@ -151,7 +152,9 @@ __global__ __launch_bounds__(kTileScanBlock) void tile_scan_kernel(uint32_t* __r
 // the next step as they are read.
 __global__ __launch_bounds__(kSortBlock) void cell_scan_kernel(uint32_t* __restrict__ count,
                                                                uint32_t* __restrict__ cstart, uint32_t C,
-                                                               uint32_t* __restrict__ queue) {
+                                                               uint32_t* __restrict__ queue,
+                                                               const uint32_t* __restrict__ halt) {
+    if (halt && *halt) return;
     __shared__ uint32_t s_tot[kSortWaves];
     const uint32_t b = blockIdx.x;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -248,7 +251,9 @@ __global__ __launch_bounds__(kSortBlock) void cell_scan_kernel(uint32_t* __restr
 
 __global__ void cell_scatter_kernel(uint32_t n, uint32_t C, const uint32_t* __restrict__ key0,
                                     const uint32_t* __restrict__ rank0, const uint32_t* __restrict__ cstart,
-                                    uint32_t* __restrict__ tmp, uint32_t* __restrict__ err) {
+                                    uint32_t* __restrict__ tmp, uint32_t* __restrict__ err,
+                                    const uint32_t* __restrict__ halt) {
+    if (halt && *halt) return;
     const uint32_t k = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const uint32_t key = key0[k];
@@ -266,7 +271,9 @@ __global__ void cell_rank_kernel(uint32_t n, uint32_t C, const uint32_t* __restr
                                  float4* __restrict__ vel_out, uint32_t* __restrict__ id_out,
                                  uint32_t* __restrict__ skey, uint32_t* __restrict__ perm,
                                  uint32_t* __restrict__ tsum, uint32_t tsum_words,
-                                 uint32_t* __restrict__ err, float* __restrict__ soa) {
+                                 uint32_t* __restrict__ err, float* __restrict__ soa,
+                                 const uint32_t* __restrict__ halt) {
+    if (halt && *halt) return;
     // The scan has read the tile sums; clear them for the integrate pass.
     if (blockIdx.x == 0)
         for (uint32_t i = threadIdx.x; i < tsum_words; i += blockDim.x) tsum[i] = 0u;
@@ -494,14 +501,16 @@ hipError_t launch_sort_keys(hipStream_t st, const SortPlan& sp) {
     // The cell-start table (both modes).
     const uint32_t stiles = scan_tiles(sp.ncells);
     if (stiles > kInlineTiles)
-        hipLaunchKernelGGL(tile_scan_kernel, dim3(1), dim3(kTileScanBlock), 0, st, sp.count, sp.ncells, stiles);
+        hipLaunchKernelGGL(tile_scan_kernel, dim3(1), dim3(kTileScanBlock), 0, st, sp.count, sp.ncells, stiles,
+                           sp.halt);
     hipLaunchKernelGGL(cell_scan_kernel, dim3(stiles), dim3(kSortBlock), 0, st, sp.count, sp.cstart, sp.ncells,
-                       sp.queue);
+                       sp.queue, sp.halt);
     if (sp.mode == kSortCellCount) {
         hipLaunchKernelGGL(cell_scatter_kernel, dim3(blocks_for(sp.n, 256)), dim3(256), 0, st, sp.n, sp.ncells,
-                           sp.key0, sp.rank0, sp.cstart, sp.tmp, sp.err);
+                           sp.key0, sp.rank0, sp.cstart, sp.tmp, sp.err, sp.halt);
         return hipGetLastError();
     }
+    if (sp.halt) return hipErrorInvalidValue;  // the frame loop runs the cell-count sort only
     const uint32_t tiles = sort_tiles(sp.n, sp.items);
     const uint32_t hist_blocks = (sp.n + kSortBlock * kHistItems - 1) / (kSortBlock * kHistItems);
     const uint32_t status_words = (uint32_t)sp.passes * tiles * kBins;
@@ -545,7 +554,7 @@ hipError_t launch_sort_gather(hipStream_t st, const SortPlan& sp) {
         hipLaunchKernelGGL(cell_rank_kernel, dim3(blocks_for(sp.n, sp.reorder_block)), dim3(sp.reorder_block), 0,
                            st, sp.n, sp.ncells, sp.key0, sp.cstart, sp.tmp, sp.pos_in, sp.vel_in, sp.id_in,
                            sp.pos_out, sp.vel_out, sp.id_out, sp.skey, sp.perm, tsum,
-                           tsum_words, sp.err, sp.soa_out);
+                           tsum_words, sp.err, sp.soa_out, sp.halt);
     else
         hipLaunchKernelGGL(radix_reorder_kernel, dim3(blocks_for(sp.n, sp.reorder_block)), dim3(sp.reorder_block),
                            0, st, sp.n, sp.perm, sp.pos_in, sp.vel_in, sp.id_in, sp.pos_out, sp.vel_out,

@@ -161,6 +161,7 @@ __device__ __forceinline__ uint32_t force_one(const StepArgs& a, const SceneCons
 
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void density_kernel(StepArgs a) {
+    if (a.halt && *a.halt) return;
     const __amdgpu_buffer_rsrc_t rpos = buffer_of(a.pos_s, a.n);
     const __amdgpu_buffer_rsrc_t rx = buffer_of(a.soa, a.n);
     const __amdgpu_buffer_rsrc_t ry = buffer_of(a.soa + a.n, a.n);
@@ -178,6 +179,7 @@ __global__ __launch_bounds__(BLOCK) void density_kernel(StepArgs a) {
 // the seven the register count would otherwise allow.
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8, 8))) void force_kernel(StepArgs a) {
+    if (a.halt && *a.halt) return;
     const __amdgpu_buffer_rsrc_t rpos = buffer_of(a.pos_s, a.n);
     const __amdgpu_buffer_rsrc_t rvel = buffer_of(a.vel_s, a.n);
     const __amdgpu_buffer_rsrc_t rcs = buffer_of(a.cstart, a.ncells + 1u);
@@ -267,17 +269,37 @@ __global__ void neighbors_kernel(StepArgs a, const uint32_t* __restrict__ offset
 }
 
 // dt = min(dt_max, (cfl h) / (c0 + |v|max)) per scene, in this op order.
+// With a frame state: nothing once halted; else scene 0's step is added to
+// the frame time (float, as the host frame loop adds it) and a frame of at
+// least 1/30 s marks pending.
 __global__ void cfl_kernel(uint32_t nscenes, SceneConst* __restrict__ scenes, uint32_t* __restrict__ vmax2,
-                           float* __restrict__ dt_out) {
+                           float* __restrict__ dt_out, uint32_t* __restrict__ frame) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nscenes) return;
+    if (frame && frame[0]) return;
     SceneConst& sc = scenes[s];
     if (sc.cfl > 0.0f) {
         const float vmax = sqrtf(__uint_as_float(vmax2[s]));
         sc.dt = fminf(sc.dt_max, (sc.cfl * sc.h) / (sc.c0 + vmax));
     }
     vmax2[s] = 0u;
-    if (s == 0) *dt_out = sc.dt;
+    if (s == 0) {
+        *dt_out = sc.dt;
+        if (frame) {
+            const float ft = __uint_as_float(frame[3]) + sc.dt;
+            frame[3] = __float_as_uint(ft);
+            frame[2] += 1u;
+            if (!((double)ft < 0.0333333333)) frame[1] = 1u;
+        }
+    }
+}
+
+__global__ void frame_reset_kernel(uint32_t* __restrict__ frame) {
+    if (threadIdx.x < (uint32_t)kFrameWords) frame[threadIdx.x] = 0u;
+}
+
+__global__ void frame_gate_kernel(uint32_t* __restrict__ frame) {
+    if (threadIdx.x == 0 && frame[1]) frame[0] = 1u;
 }
 
 __global__ void fill_u32_kernel(uint32_t* __restrict__ p, uint32_t v, size_t count) {
@@ -353,9 +375,21 @@ hipError_t launch_neighbors(hipStream_t st, int block, uint32_t grid, const Step
     return hipGetLastError();
 }
 
-hipError_t launch_cfl(hipStream_t st, uint32_t nscenes, SceneConst* scenes, uint32_t* vmax2, float* dt_out) {
+hipError_t launch_cfl(hipStream_t st, uint32_t nscenes, SceneConst* scenes, uint32_t* vmax2, float* dt_out,
+                      uint32_t* frame) {
     if (nscenes == 0) return hipSuccess;
-    hipLaunchKernelGGL(cfl_kernel, dim3((nscenes + 255) / 256), dim3(256), 0, st, nscenes, scenes, vmax2, dt_out);
+    hipLaunchKernelGGL(cfl_kernel, dim3((nscenes + 255) / 256), dim3(256), 0, st, nscenes, scenes, vmax2, dt_out,
+                       frame);
+    return hipGetLastError();
+}
+
+hipError_t launch_frame_reset(hipStream_t st, uint32_t* frame) {
+    hipLaunchKernelGGL(frame_reset_kernel, dim3(1), dim3(64), 0, st, frame);
+    return hipGetLastError();
+}
+
+hipError_t launch_frame_gate(hipStream_t st, uint32_t* frame) {
+    hipLaunchKernelGGL(frame_gate_kernel, dim3(1), dim3(64), 0, st, frame);
     return hipGetLastError();
 }
 

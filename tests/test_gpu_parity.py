@@ -439,6 +439,34 @@ def test_cell_subdivision_parity(E, xsub):
     step_and_compare(E, e, cfg["params"], f"xsub {xsub}")
 
 
+def test_cfl_frames_on_device_match_the_host_loop(E):
+    """Adaptive step: sph_run_frame replays a step graph that stops itself
+    once the frame reaches 1/30 s (reading the frame state once per batch);
+    frame times, step counts and the state equal the step-by-step host loop
+    bit for bit, including the first frame after an upload."""
+    import time
+    p = E.default_params(24)
+    p.dt = 5e-3            # the CFL rule sets the step
+    p.cfl_number = 0.4
+    pos = E.scene_generate(E.SCENE_DAMBREAK, 24)
+    res = {}
+    for graph in (True, False):
+        e = make_engine(E, p, pos)
+        e.set_use_graph(graph)
+        frames = [e.run_frame() for _ in range(3)]
+        e.synchronize()
+        t0 = time.perf_counter()
+        frames += [e.run_frame() for _ in range(5)]
+        ms = (time.perf_counter() - t0) / 5 * 1e3
+        res[graph] = (frames, e.get_particles(), e.get_velocity())
+        say(f"[cfl frames] device loop {graph}: {ms:.2f} ms per frame, steps {[f[1] for f in frames]}")
+        e.close()
+    assert res[True][0] == res[False][0]
+    assert all(t >= np.float32(0.0333333333) for t, _ in res[True][0])
+    assert len(set(k for _, k in res[True][0])) > 1   # the step length varies
+    assert np.array_equal(res[True][1], res[False][1]) and np.array_equal(res[True][2], res[False][2])
+
+
 def test_cfl_step_parity(E):
     """Adaptive step: the engine's step lengths follow the oracle's CFL rule
     (identical first step from rest, then within 1e-4 as the trajectories
