@@ -1,6 +1,6 @@
 // TEST INFRASTRUCTURE ONLY: reverse-mode restatement of the step in
 // sph_oracle.hpp, used to pin the engine's hand-written adjoint.  The
-// reference has no differentiable path (north-star addition); this oracle is
+// reference has no differentiable path (the adjoint is new here); this oracle is
 // itself pinned by FP64 central differences and by torch float64 autograd of a
 // dense pairwise restatement (tests/).
 //

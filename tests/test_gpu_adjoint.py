@@ -1,6 +1,6 @@
 """GPU hand-written adjoint vs FP64 central differences, the CPU reverse-mode
-oracle and a torch.autograd wrapper (within 1e-3 relative of
-FP64 finite differences at 4,096 particles x 50 steps)."""
+oracle and a torch.autograd wrapper (bar: within 1e-3 relative of FP64 finite
+differences at 4,096 particles x 50 steps)."""
 import numpy as np
 import pytest
 
