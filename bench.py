@@ -368,7 +368,9 @@ def adjoint(args):
     vel0 = np.concatenate([s["vel"] for s in sc])
     b.set_particles(pos0, vel0)
     b.make_ready()
-    b.rollout_forward(10, 5)  # warm-up of both passes
+    # Warm-up of both passes on the timed shape, so that neither timed region
+    # allocates its checkpoints or tape.
+    b.rollout_forward(rc["steps"], rc["checkpoint_every"])
     b.rollout_backward(np.zeros((len(pos0), 3), np.float32))
     b.set_particles(pos0, vel0)
     b.make_ready()

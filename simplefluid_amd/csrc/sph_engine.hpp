@@ -104,6 +104,7 @@ struct RolloutState {
     bool valid = false;
     uint32_t n = 0;                                // particles of the rollout
     uint64_t gen = 0;                              // handle configuration it was run on
+    bool last_taped = false;                       // the tape holds the last segment (forward recorded it)
     std::vector<std::unique_ptr<StepBufs>> tape;   // `every` slots
     DevBuf<float4> ck_pos, ck_vel;                 // (nseg + 1) x n; slot nseg = final state
     DevBuf<uint32_t> ck_id;

@@ -52,21 +52,40 @@ int sph_rollout_forward(sph_handle* h, uint32_t steps, uint32_t every) {
     R.steps = steps;
     R.every = std::min(every, steps);
     R.nseg = (steps + R.every - 1) / R.every;
+    R.last_taped = false;
     const size_t n = h->n;
     hipError_t e = R.ck_pos.ensure((R.nseg + 1) * n);
     if (e == hipSuccess) e = R.ck_vel.ensure((R.nseg + 1) * n);
     if (e == hipSuccess) e = R.ck_id.ensure((R.nseg + 1) * n);
     if (e != hipSuccess) return hip_fail(h, e, "checkpoint allocation");
+    // One segment's tape (each step's sorted state, cells, density, masks,
+    // acceleration); the last segment is recorded on it as it runs, so the
+    // reverse pass starts without recomputing that segment.
+    while (R.tape.size() < R.every) R.tape.emplace_back(new StepBufs());
+    for (uint32_t t = 0; t < R.every; ++t) {
+        const int rc = alloc_step_bufs(h, *R.tape[t]);
+        if (rc != SPH_OK) return rc;
+        e = R.tape[t]->acc.ensure(n);
+        if (e != hipSuccess) return hip_fail(h, e, "tape allocation");
+    }
     for (uint32_t s = 0; s < R.nseg; ++s) {
         e = save_checkpoint(h, R, s);
         if (e != hipSuccess) return hip_fail(h, e, "checkpoint copy");
         const uint32_t len = std::min(R.every, steps - s * R.every);
-        const int rc = sph_advance_steps(h, len);
-        if (rc != SPH_OK) return rc;
+        if (s + 1 < R.nseg) {
+            const int rc = sph_advance_steps(h, len);
+            if (rc != SPH_OK) return rc;
+        } else {
+            for (uint32_t t = 0; t < len; ++t) {
+                const int rc = enqueue_step(h, *R.tape[t], false);
+                if (rc != SPH_OK) return rc;
+            }
+        }
     }
     e = save_checkpoint(h, R, R.nseg);
     if (e != hipSuccess) return hip_fail(h, e, "checkpoint copy");
     R.valid = true;
+    R.last_taped = true;
     R.n = h->n;
     R.gen = h->config_gen;
     return SPH_OK;
@@ -117,12 +136,16 @@ int sph_rollout_backward(sph_handle* h, const float* dL_dx_final, float* dL_dB, 
 
     const uint32_t grid = ntiles;  // one block per tile
     for (int seg = (int)R.nseg - 1; seg >= 0; --seg) {
-        chk(load_checkpoint(h, R, (uint32_t)seg));
-        if (e != hipSuccess) return hip_fail(h, e, "checkpoint restore");
         const uint32_t len = std::min(R.every, R.steps - (uint32_t)seg * R.every);
-        for (uint32_t t = 0; t < len; ++t) {
-            const int rc = enqueue_step(h, *R.tape[t], false);
-            if (rc != SPH_OK) return rc;
+        if (seg == (int)R.nseg - 1 && R.last_taped) {
+            R.last_taped = false;  // the tape is about to hold earlier segments
+        } else {
+            chk(load_checkpoint(h, R, (uint32_t)seg));
+            if (e != hipSuccess) return hip_fail(h, e, "checkpoint restore");
+            for (uint32_t t = 0; t < len; ++t) {
+                const int rc = enqueue_step(h, *R.tape[t], false);
+                if (rc != SPH_OK) return rc;
+            }
         }
         for (int t = (int)len - 1; t >= 0; --t) {
             StepBufs& w = *R.tape[t];
