@@ -194,6 +194,10 @@ int alloc_step_bufs(sph_handle* h, StepBufs& w) {
     chk(w.skey.ensure(n));
     chk(w.perm.ensure(n));
     chk(w.cstart.ensure((size_t)h->ncells + 1));
+    // The table's contents are for another grid (or none) now: every tile
+    // is rewritten at the next scan.
+    chk(w.tile_pre.ensure(scan_tiles(h->ncells)));
+    if (e == hipSuccess) chk(launch_fill_u32(h->stream, w.tile_pre.p, 0xffffffffu, scan_tiles(h->ncells)));
     if (h->tune.sort_mode == kSortCellCount) {
         chk(w.tmp.ensure(n));
     } else {
@@ -282,6 +286,7 @@ int enqueue_step(sph_handle* h, StepBufs& w, bool record_prof, uint32_t* frame) 
     sp.rank0 = h->rank0.p;
     sp.count = h->count.p;
     sp.cstart = w.cstart.p;
+    sp.tile_pre = w.tile_pre.p;
     sp.tmp = w.tmp.p;
     for (int i = 0; i < 2; ++i) { sp.kbuf[i] = w.kbuf[i].p; sp.vbuf[i] = w.vbuf[i].p; }
     sp.skey = w.skey.p;

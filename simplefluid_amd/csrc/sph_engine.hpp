@@ -78,6 +78,7 @@ struct StepBufs {
     DevBuf<uint32_t> id_s;
     DevBuf<uint32_t> skey, perm;   // sorted keys, sorted index -> state index
     DevBuf<uint32_t> cstart;       // C + 1 lower-bound cell-start table
+    DevBuf<uint32_t> tile_pre;     // per scan tile of cstart: prefix held by an empty tile, or ~0u
     DevBuf<uint32_t> tmp;          // cell-count sort slots
     DevBuf<uint32_t> kbuf[2], vbuf[2];  // radix ping-pong (radix mode only)
     DevBuf<float> rho;

@@ -24,6 +24,8 @@ struct SortPlan {
     const uint32_t* rank0 = nullptr;  // slot each particle took in its cell's count
     uint32_t* count = nullptr;        // C cell counts (zeroed by the scan) + per-tile sums
     uint32_t* cstart = nullptr;       // C + 1, written by the scan
+    uint32_t* tile_pre = nullptr;     // per scan tile: the prefix this cstart holds in all its cells
+                                      // when the tile was empty at its last write, else 0xffffffff
     uint32_t* tmp = nullptr;          // cell-count: slot -> particle
     uint32_t* kbuf[2] = {nullptr, nullptr};  // radix ping-pong
     uint32_t* vbuf[2] = {nullptr, nullptr};

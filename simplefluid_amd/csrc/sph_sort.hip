@@ -153,7 +153,8 @@ __global__ __launch_bounds__(kTileScanBlock) void tile_scan_kernel(uint32_t* __r
 __global__ __launch_bounds__(kSortBlock) void cell_scan_kernel(uint32_t* __restrict__ count,
                                                                uint32_t* __restrict__ cstart, uint32_t C,
                                                                uint32_t* __restrict__ queue,
-                                                               const uint32_t* __restrict__ halt) {
+                                                               const uint32_t* __restrict__ halt,
+                                                               uint32_t* __restrict__ tile_pre) {
     if (halt && *halt) return;
     __shared__ uint32_t s_tot[kSortWaves];
     const uint32_t b = blockIdx.x;
@@ -174,7 +175,16 @@ __global__ __launch_bounds__(kSortBlock) void cell_scan_kernel(uint32_t* __restr
     } else {
         pre = count[C + T + b];
     }
+    // What the table holds for this tile, read by every thread before any
+    // thread of the block rewrites it.
+    const uint32_t held = tile_pre[b];
+    __syncthreads();
     if (count[C + b] == 0u) {
+        // An empty tile's cells all hold its prefix; skip the rewrite when
+        // this table already holds that prefix there (most empty tiles of
+        // a sparse grid keep theirs from step to step).
+        if (held == pre) return;
+        if (tid == 0) tile_pre[b] = pre;
         const uint32_t t0 = b << kScanTileShift;
         const uint32_t t1 = t0 + (1u << kScanTileShift) < C ? t0 + (1u << kScanTileShift) : C;
         const uint4 v4 = make_uint4(pre, pre, pre, pre);
@@ -188,6 +198,7 @@ __global__ __launch_bounds__(kSortBlock) void cell_scan_kernel(uint32_t* __restr
         if (t1 == C && tid == 0) cstart[C] = pre;
         return;
     }
+    if (tid == 0) tile_pre[b] = 0xffffffffu;
     const uint32_t base = (b << kScanTileShift) + (uint32_t)wave * 1024u;
     uint4 v[4];
 #pragma unroll
@@ -504,7 +515,7 @@ hipError_t launch_sort_keys(hipStream_t st, const SortPlan& sp) {
         hipLaunchKernelGGL(tile_scan_kernel, dim3(1), dim3(kTileScanBlock), 0, st, sp.count, sp.ncells, stiles,
                            sp.halt);
     hipLaunchKernelGGL(cell_scan_kernel, dim3(stiles), dim3(kSortBlock), 0, st, sp.count, sp.cstart, sp.ncells,
-                       sp.queue, sp.halt);
+                       sp.queue, sp.halt, sp.tile_pre);
     if (sp.mode == kSortCellCount) {
         hipLaunchKernelGGL(cell_scatter_kernel, dim3(blocks_for(sp.n, 256)), dim3(256), 0, st, sp.n, sp.ncells,
                            sp.key0, sp.rank0, sp.cstart, sp.tmp, sp.err, sp.halt);
