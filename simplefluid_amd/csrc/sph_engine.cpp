@@ -596,6 +596,14 @@ static int capture_graph(sph_handle* h, bool gated = false) {
     if (gr) (void)hipGraphDestroy(gr);
     gx = nullptr;
     gr = nullptr;
+    // Tile tables are built on first use with a synchronous copy, which a
+    // capturing stream does not allow: build the ones the step uses first
+    // (a block size set after steps have run has none yet).
+    const int blocks[] = {h->tune.density_block, h->tune.force_block};
+    for (int b : blocks) {
+        uint32_t nt = 0;
+        if (!tiles_for(h, b, nt)) return fail(h, SPH_E_OOM, "tile table");
+    }
     hipError_t e = hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return hip_fail(h, e, "begin capture");
     const int rc = enqueue_step(h, h->work, false, gated ? h->frame.p : nullptr);

@@ -481,6 +481,27 @@ def test_cfl_frames_on_device_match_the_host_loop(E):
     assert np.array_equal(res[True][1], res[False][1]) and np.array_equal(res[True][2], res[False][2])
 
 
+def test_tuning_changes_between_captured_frames(E):
+    """Block sizes changed between frames of the adaptive step's device loop
+    (each change re-captures the step graph, with tile tables for block sizes
+    not used before) give the bits of a run with fixed tuning."""
+    p = E.default_params(24)
+    p.dt = 5e-3
+    p.cfl_number = 0.4
+    pos = E.scene_generate(E.SCENE_DAMBREAK, 24)
+    res = []
+    for plan in (((256, 256, 256, 8),), ((64, 128, 256, 8), (512, 64, 256, 8, 1), (128, 512, 256, 8))):
+        e = make_engine(E, p, pos)
+        frames = []
+        for i in range(6):
+            e.set_tuning(*plan[i % len(plan)])
+            frames.append(e.run_frame())
+        res.append((frames, e.get_particles(), e.get_velocity()))
+        e.close()
+    assert res[0][0] == res[1][0]
+    assert np.array_equal(res[0][1], res[1][1]) and np.array_equal(res[0][2], res[1][2])
+
+
 def test_cfl_step_parity(E):
     """Adaptive step: the engine's step lengths follow the oracle's CFL rule
     (identical first step from rest, then within 1e-4 as the trajectories
