@@ -195,6 +195,33 @@ def test_batched_adjoint_equals_single_scene():
         assert np.array_equal(dv0[off[i]:off[i + 1]], sdv0.astype(np.float32))
 
 
+def test_batched_adjoint_with_an_empty_scene():
+    """An empty scene in a batch gets zero parameter gradients and leaves the
+    other scenes' gradients bit for bit as without it."""
+    from simplefluid_amd import engine
+    cases = [case(n3=(8, 10, 8), seed=s, B=40.0 + 20 * s, mu=0.01 * (s + 1)) for s in range(2)]
+    steps, every = 12, 5
+    res = []
+    for layout in ((0, 1), (0, None, 1)):
+        ps = [cases[i][0] if i is not None else cases[0][0] for i in layout]
+        counts = [len(cases[i][1]) if i is not None else 0 for i in layout]
+        b = engine.BatchEngine(ps, counts)
+        b.set_cell_subdivision(4)
+        b.set_particles(np.concatenate([cases[i][1] for i in layout if i is not None]),
+                        np.concatenate([cases[i][2] for i in layout if i is not None]))
+        b.make_ready()
+        b.rollout_forward(steps, every)
+        xT = b.get_particles().astype(np.float64)
+        tgt = np.concatenate([cases[i][3] for i in layout if i is not None])
+        dB, dmu, dv0 = b.rollout_backward((2e-3 * (xT - tgt)).astype(np.float32))
+        res.append((np.asarray(dB), np.asarray(dmu), dv0, xT))
+        b.close()
+    (dB0, dmu0, dv00, x0), (dB1, dmu1, dv01, x1) = res
+    assert np.array_equal(x0, x1) and np.array_equal(dv00, dv01)
+    assert dB1[1] == 0.0 and dmu1[1] == 0.0
+    assert np.array_equal(dB0, dB1[[0, 2]]) and np.array_equal(dmu0, dmu1[[0, 2]])
+
+
 def test_torch_autograd_function():
     import torch
     import oracle

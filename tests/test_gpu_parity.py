@@ -776,6 +776,30 @@ def test_batched_equals_per_scene(E):
         e.close()
 
 
+def test_batched_with_empty_scenes(E):
+    """Scenes with no particles (between others and last) own no tiles and
+    change nothing: the other scenes step bit for bit as in a batch without
+    them, and the empty ones report empty ranges."""
+    from simplefluid_amd import configs
+    ens = configs.ensemble(nscenes=2, seed=17, nmin=3000, nmax=9000)
+    params = [s["params"] for s in ens["scenes"]]
+    pos = [s["pos"] for s in ens["scenes"]]
+    out = []
+    for layout in ((0, 1), (0, None, 1, None)):
+        ps = [params[i] if i is not None else params[0] for i in layout]
+        counts = [len(pos[i]) if i is not None else 0 for i in layout]
+        b = E.BatchEngine(ps, counts)
+        b.set_cell_subdivision(2)
+        b.set_particles(np.concatenate([pos[i] for i in layout if i is not None]))
+        b.make_ready()
+        b.advance_steps(30)
+        off = b.scene_offsets()
+        assert [int(off[k + 1] - off[k]) for k in range(len(layout))] == counts
+        out.append((b.get_particles(), b.get_velocity()))
+        b.close()
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
 def test_batched_persistent_queues_match(E):
     """A batch of six ~100k-particle scenes (more 64-lane tiles than resident
     blocks, so the per-XCD tile queues are in use) gives the same bits with
