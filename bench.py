@@ -426,9 +426,9 @@ def main():
                     help="run a 256 MiB device copy (known bytes) for counter calibration, then exit")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported by this build "
-              "(the batched ensemble would shard by scene with no communication, the plain "
-              "path by spatial slabs with a ghost band)", file=sys.stderr)
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported; the engine runs "
+              "on one GPU (the batched ensemble would shard by scene with no communication, the "
+              "plain path by spatial slabs with a ghost band)", file=sys.stderr)
         sys.exit(2)
     if args.calibrate_copy:
         calibrate_copy()
