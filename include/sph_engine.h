@@ -214,7 +214,9 @@ int  sph_get_sort_mode(const sph_handle* h);
  * counterpart).  xsub = 0 picks automatically (the largest of 4, 2, 1 with
  * at most 12 cells per particle, or 2^20 cells);
  * 1, 2, 4 force it.  Finer x cells let each neighbour row skip the x cells
- * outside the support sphere's chord.                                       */
+ * outside the support sphere's chord.  Limits: 1024 cells per axis (times
+ * xsub in x), 2^30 - 2 cells per handle, fewer than 2^28 particles; larger
+ * setups return SPH_E_INVALID.                                              */
 int  sph_set_cell_subdivision(sph_handle* h, int xsub);
 
 /* ---- test-only views (parity against the CPU oracle) ---- */

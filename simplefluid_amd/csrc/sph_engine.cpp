@@ -53,6 +53,10 @@ static bool finite3(const float* v) { return std::isfinite(v[0]) && std::isfinit
 
 // Derived constants, formed exactly as the CPU restatement forms them:
 // normalisations in double from the float inputs, rounded once to float.
+// The passes read the C + 1 cell-start words through buffer resources,
+// whose byte range and offsets are 32-bit: 4 (C + 1) must stay below 2^32.
+constexpr uint64_t kMaxCells = (1ull << 30) - 2;
+
 int derive_scene(const sph_params& p, SceneConst& sc, std::string& err, int xsub) {
     if (!(p.particle_radius > 0.f) || !std::isfinite(p.particle_radius)) { err = "particle_radius must be > 0"; return SPH_E_INVALID; }
     if (!(p.kernel_radius > 0.f) || !std::isfinite(p.kernel_radius)) { err = "kernel_radius must be > 0"; return SPH_E_INVALID; }
@@ -113,7 +117,7 @@ int derive_scene(const sph_params& p, SceneConst& sc, std::string& err, int xsub
         sc.hi[k] = p.box_max[k] - r;
         cells *= (uint64_t)G;
     }
-    if (cells > (1ull << 30)) { err = "too many grid cells"; return SPH_E_INVALID; }
+    if (cells > kMaxCells) { err = "too many grid cells (limit 2^30 - 2)"; return SPH_E_INVALID; }
     return SPH_OK;
 }
 
@@ -407,7 +411,7 @@ static int configure(sph_handle* h, const std::vector<sph_params>& ps, const std
         sc[s].p_end = (uint32_t)p_off;
         off[s + 1] = (uint32_t)p_off;
     }
-    if (cell_off > (1ull << 30)) return fail(h, SPH_E_INVALID, "too many grid cells in total");
+    if (cell_off > kMaxCells) return fail(h, SPH_E_INVALID, "too many grid cells in total (limit 2^30 - 2)");
     if (p_off >= (1ull << 28)) return fail(h, SPH_E_INVALID, "too many particles (limit 2^28)");
     h->params = ps;
     h->scene_n = ns;

@@ -688,6 +688,39 @@ def test_single_particle_and_empty_input(E):
     z.close()
 
 
+def test_largest_grid_steps_and_a_larger_one_is_refused(E):
+    """Maximum grid: 1024 x 1024 x 1023 cells, just under the 2^30 - 2 limit
+    that keeps every cell-start offset inside 32-bit buffer addressing.  A
+    close pair and a lone particle at the far corner (the top of the
+    cell-start table) step bit for bit as the same particles do in a small
+    box with the same upper corner (two-term sums do not depend on the
+    order); a 1024^3 grid is refused when the handle is created."""
+    h = 0.001
+    cs = h * (1.0 + 2.0 ** -10)
+    hi = (1023.5 * cs, 1023.5 * cs, 1022.5 * cs)
+    c = np.array(hi) - 10 * h
+    pos = np.array([c, c + [0.6 * h, 0.0, 0.0], c - [0.0, 0.0, 20 * h]], np.float32)
+    vel = np.array([[0.1, 0.0, 0.0], [-0.1, 0.05, 0.0], [0.0, 0.0, 0.2]], np.float32)
+    kw = dict(kernel_radius=h, particle_radius=h / 4, pressure_stiffness=50.0, dt=1e-4)
+    res = []
+    for lo in ((0.0, 0.0, 0.0), tuple(np.array(hi) - 40 * h)):
+        e = E.Engine(E.make_params(box_min=lo, box_max=hi, **kw))
+        e.set_cell_subdivision(1)
+        e.set_particles(pos, vel)
+        e.make_ready()
+        if lo == (0.0, 0.0, 0.0):
+            assert int(np.prod(np.asarray(e.debug_constants()["G"], np.int64))) == 1024 * 1024 * 1023
+        e.advance_steps(20)
+        res.append((e.get_particles(), e.get_velocity(), e.get_density()))
+        e.close()
+    for a, b in zip(res[0], res[1]):
+        assert np.array_equal(a, b)
+    assert res[0][2][0] > res[0][2][2]   # the pair sees each other, the lone particle only itself
+    with pytest.raises(E.SPHError) as ei:
+        E.Engine(E.make_params(box_min=(0.0, 0.0, 0.0), box_max=(hi[0], hi[1], hi[0]), **kw))
+    assert ei.value.code == E.SPH_E_INVALID and "grid cells" in str(ei.value)
+
+
 def test_device_memory_limit(E):
     """A handle's buffers are charged to it; an allocation past its cap fails
     with SPH_E_OOM and a message naming the limit, and the handle works once
