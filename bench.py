@@ -360,6 +360,8 @@ def adjoint(args):
     velocities; gradients for stiffness, viscosity and initial velocity."""
     from simplefluid_amd import configs, engine
     rc = configs.rollout_scenes()
+    if args.checkpoint_every:
+        rc["checkpoint_every"] = args.checkpoint_every
     sc = rc["scenes"]
     counts = [s["pos"].shape[0] for s in sc]
     target = rollout_targets(rc)
@@ -419,6 +421,9 @@ def main():
     ap.add_argument("--only", choices=["plain", "batched", "adjoint", "dambreak65k", "sphere4m"], default=None,
                     help="run one leg alone (profiling)")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--checkpoint-every", type=int, default=0,
+                    help="adjoint leg: steps between checkpoints (default: the workload's 20; "
+                         "200 tapes every step and recomputes nothing)")
     ap.add_argument("--tuning", type=str, default="",
                     help="density,force,reorder blocks, sort items [,persistent 0/1], e.g. 256,128,512,4,0 "
                          "(skips autotune)")
