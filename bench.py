@@ -247,10 +247,12 @@ def batched(args):
     k = args.batch_steps
     tb = time_steps(b, k)
     vb = N * k / tb
+    tuning = b.get_tuning()
     b.close()
     if args.no_per_scene:
         log(f"batched: {N} particles in 256 scenes, {k} steps: {vb:.4e}")
-        return {"scenes": 256, "particles": N, "steps": k, "value": round(vb, 1), "unit": "particle-steps/s"}
+        return {"scenes": 256, "particles": N, "steps": k, "value": round(vb, 1), "unit": "particle-steps/s",
+                "tuning": tuning}
     # Per-scene launches: one engine per scene, stepped scene by scene.
     engines = []
     for s in ens["scenes"]:
@@ -273,7 +275,7 @@ def batched(args):
     log(f"batched: {N} particles in 256 scenes, {k} steps: {vb:.4e} vs per-scene {vs:.4e}")
     return {"scenes": 256, "particles": N, "steps": k, "value": round(vb, 1),
             "per_scene_value": round(vs, 1), "speedup": round(vb / vs, 3),
-            "unit": "particle-steps/s"}
+            "unit": "particle-steps/s", "tuning": tuning}
 
 
 def dambreak_65k_leg(args):
