@@ -241,7 +241,9 @@ def batched(args):
     b = engine.BatchEngine(params, counts)
     b.set_particles(np.concatenate([s["pos"] for s in ens["scenes"]]))
     b.make_ready()
-    if not args.no_autotune:
+    if args.tuning:
+        b.set_tuning(*[int(x) for x in args.tuning.split(",")])
+    elif not args.no_autotune:
         b.autotune()
     b.advance_steps(5)
     k = args.batch_steps
