@@ -15,6 +15,10 @@
 //   qbar_i  = cV q_j psi (abar_i-abar_j).(v_j-v_i)
 //   rhobar_i = Pbar_i (p' q^2 - 2 p q^3) - qbar_i q^2
 //   Bbar += Pbar_i q_i^2 dp/dB ;  mubar += cP abar_i . q_i sum_j psi q_j (v_j - v_i)
+// Walls (per axis d, u_lo = (x_d - box_min_d)/h, u_hi = (box_max_d - x_d)/h):
+//   Pbar_i += cW abar_i,d (w_j(u_lo) - w_j(u_hi))
+//   xbar_i,d += cW P_i abar_i,d (w_j'(u_lo) + w_j'(u_hi)) / h
+//             + rhobar_i rho0 (w_rho'(u_lo) - w_rho'(u_hi)) / h
 // The clamp passes xbar through only on unclamped axes and scales vbar by
 // -restitution where the velocity was reflected.
 #pragma once
@@ -113,6 +117,16 @@ void reverse_step(const Consts<T>& c, const Tape<T>& tp, uint32_t n, std::vector
             qb += cV * qj * hr * dav;
             Avx += hr * qj * dvx; Avy += hr * qj * dvy; Avz += hr * qj * dvz;
         }
+        {
+            const T pk[3] = {xi, yi, zi}, ab3[3] = {abx, aby, abz};
+            T gw[3];
+            for (int d = 0; d < 3; ++d) {
+                const T ul = (pk[d] - c.wmin[d]) * c.inv_h, uh = (c.wmax[d] - pk[d]) * c.inv_h;
+                Pb += c.cW * ab3[d] * (wall_j(ul) - wall_j(uh));
+                gw[d] = ((c.cW * Pi) * ab3[d]) * (c.inv_h * (wall_j_du(ul) + wall_j_du(uh)));
+            }
+            gx += gw[0]; gy += gw[1]; gz += gw[2];
+        }
         T p, dp_drho, dp_dB;
         eos_grad(c, tp.rho[i], p, dp_drho, dp_dB);
         rhobar[i] = Pb * (dp_drho * qi * qi - T(2) * p * qi * qi * qi) - qb * qi * qi;
@@ -134,6 +148,15 @@ void reverse_step(const Consts<T>& c, const Tape<T>& tp, uint32_t n, std::vector
             const T t = h2 - dist2(rx, ry, rz);
             const T s = T(-6) * cD * t * t * (rhobar[i] + rhobar[j]);
             gx += s * rx; gy += s * ry; gz += s * rz;
+        }
+        {
+            const T pk[3] = {xi, yi, zi};
+            T gw[3];
+            for (int d = 0; d < 3; ++d) {
+                const T ul = (pk[d] - c.wmin[d]) * c.inv_h, uh = (c.wmax[d] - pk[d]) * c.inv_h;
+                gw[d] = (rhobar[i] * c.rho0) * (c.inv_h * (wall_rho_du(ul) - wall_rho_du(uh)));
+            }
+            gx += gw[0]; gy += gw[1]; gz += gw[2];
         }
         xs[3 * i] += gx; xs[3 * i + 1] += gy; xs[3 * i + 2] += gz;
     }

@@ -17,6 +17,15 @@
 //     attractive-pressure option (Source/Controller.cpp:61) scales it;
 //   * symmetric pressure force with the spiky gradient, Mueller viscosity
 //     (mu/rho_i) sum m (v_j - v_i)/rho_j lap W_visc, gravity;
+//   * walls: each box face is the boundary of a half-space of fluid at rest
+//     density; a particle at distance d < h from a face gets that half-space's
+//     poly6 density, rho0 w_rho(d/h), and the pressure force of its particles
+//     taken at the particle's own p/rho^2 (mirror), n 2 (p/rho^2) (45 rho0/h)
+//     w_j(d/h), n pointing into the box, with
+//       w_rho(u) = (1-u)^5 (128 + 325u + 345u^2 + 175u^3 + 35u^4) / 256
+//                = (315/256) int_u^1 (1 - t^2)^4 dt,
+//       w_j(u)   = (1-u)^4 (1 + 4u) / 30 = int_u^1 (1-s)^2 (s^2 - u^2) ds
+//     (the half-space integrals of W_poly6 and of the spiky pressure term);
 //   * symplectic Euler and a box clamp with restitution on the normal velocity.
 // Every op sequence that decides a discrete outcome (cell index, neighbour
 // predicate) is written exactly as the engine writes it, so cells and
@@ -63,6 +72,8 @@ template <class T>
 struct Consts {
     T h, h2, inv_cs, inv_csx, mass, cD, cP, cV, inv_rho0, B, e, attr, dt, r;
     T cfl, c0, dt_max;  // CFL number, EOS sound speed sqrt(7 B / rho0), dt cap
+    T rho0, inv_h, cW;  // wall terms: rest density, 1/h, 2 * 45 rho0 / h
+    T wmin[3], wmax[3]; // box faces
     int xsub;
     T g[3], bmin[3], lo[3], hi[3];
     int G[3];
@@ -103,9 +114,14 @@ inline Consts<T> derive(const Params& p) {
     c.cfl       = (T)p.cfl_number;
     c.c0        = (T)std::sqrt(7.0 * (double)p.pressure_stiffness / (double)p.rest_density);
     c.ncells    = 1;
+    c.rho0      = (T)p.rest_density;
+    c.inv_h     = (T)(1.0 / hd);
+    c.cW        = (T)(90.0 * (double)(T)p.rest_density / hd);
     for (int k = 0; k < 3; ++k) {
         c.g[k]    = (T)p.gravity[k];
         c.bmin[k] = (T)p.box_min[k];
+        c.wmin[k] = (T)p.box_min[k];
+        c.wmax[k] = (T)p.box_max[k];
         const T ext = (T)p.box_max[k] - (T)p.box_min[k];
         int G = (int)std::ceil(ext * (k == 0 ? c.inv_csx : c.inv_cs));
         c.G[k] = G < 1 ? 1 : G;
@@ -146,6 +162,43 @@ inline T eos(const Consts<T>& c, T rho) {
     T p = c.B * (x7 - T(1));
     if (p < T(0)) p = c.attractive ? p * c.attr : T(0);
     return p;
+}
+
+// Wall profiles (u = distance to the face / h); see the file comment.  The
+// engine evaluates them with the same operations.
+template <class T>
+inline T wall_rho(T u) {
+    if (!(u < T(1))) return T(0);
+    u = std::fmax(u, T(0));
+    const T w = T(1) - u;
+    const T w2 = w * w;
+    const T w5 = (w2 * w2) * w;
+    const T poly = T(128) + u * (T(325) + u * (T(345) + u * (T(175) + u * T(35))));
+    return (w5 * poly) * T(0.00390625);
+}
+template <class T>
+inline T wall_j(T u) {
+    if (!(u < T(1))) return T(0);
+    u = std::fmax(u, T(0));
+    const T w = T(1) - u;
+    const T w2 = w * w;
+    return ((w2 * w2) * (T(1) + T(4) * u)) * T(1.0 / 30.0);
+}
+// Their derivatives in u (the reverse pass).
+template <class T>
+inline T wall_rho_du(T u) {
+    if (!(u < T(1))) return T(0);
+    u = std::fmax(u, T(0));
+    const T s = T(1) - u * u;
+    const T s2 = s * s;
+    return -T(1.23046875) * (s2 * s2);
+}
+template <class T>
+inline T wall_j_du(T u) {
+    if (!(u < T(1))) return T(0);
+    u = std::fmax(u, T(0));
+    const T w = T(1) - u;
+    return -T(2.0 / 3.0) * u * ((w * w) * w);
 }
 
 template <class T>
@@ -309,7 +362,13 @@ struct Sim {
                         }
                     }
                 }
-            const T r = c.cD * sum;
+            T wr = T(0);
+            const T pk[3] = {xi, yi, zi};
+            for (int d = 0; d < 3; ++d) {
+                wr += wall_rho((pk[d] - c.wmin[d]) * c.inv_h);
+                wr += wall_rho((c.wmax[d] - pk[d]) * c.inv_h);
+            }
+            const T r = c.cD * sum + c.rho0 * wr;
             rho[k] = r;
             const T p = eos(c, r);
             const T ir = T(1) / r;
@@ -372,9 +431,14 @@ struct Sim {
                     }
                 }
             const T cvi = c.cV * irho[k];
-            const T a0 = (c.cP * apx + cvi * avx) + c.g[0];
-            const T a1 = (c.cP * apy + cvi * avy) + c.g[1];
-            const T a2 = (c.cP * apz + cvi * avz) + c.g[2];
+            const T cwi = c.cW * pti;
+            const T pk[3] = {xi, yi, zi};
+            T aw[3];
+            for (int d = 0; d < 3; ++d)
+                aw[d] = cwi * (wall_j((pk[d] - c.wmin[d]) * c.inv_h) - wall_j((c.wmax[d] - pk[d]) * c.inv_h));
+            const T a0 = ((c.cP * apx + cvi * avx) + c.g[0]) + aw[0];
+            const T a1 = ((c.cP * apy + cvi * avy) + c.g[1]) + aw[1];
+            const T a2 = ((c.cP * apz + cvi * avz) + c.g[2]) + aw[2];
             ax[k] = a0; ay[k] = a1; az[k] = a2;
             T v[3] = {vxi + c.dt * a0, vyi + c.dt * a1, vzi + c.dt * a2};
             T p[3] = {xi + c.dt * v[0], yi + c.dt * v[1], zi + c.dt * v[2]};

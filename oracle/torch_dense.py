@@ -5,7 +5,8 @@ small particle counts; it shares no code with either.
 
 Same formulation as oracle/sph_oracle.hpp: poly6 density (self included),
 clamped Tait EOS (exponent 7), symmetric spiky pressure force, Mueller
-viscosity, gravity, symplectic Euler, box clamp with restitution.  The
+viscosity, gravity, the walls' half-space density and pressure terms,
+symplectic Euler, box clamp with restitution.  The
 neighbour predicate d^2 < h^2 is a constant mask (it is not differentiated;
 every term vanishes at d = h).
 """
@@ -21,7 +22,20 @@ def make_consts(c: dict, params) -> dict:
                 hi=[float(x) for x in c["hi"]], dt=float(params.dt),
                 e=float(params.boundary_restitution),
                 attr=float(params.attractive_pressure_ratio) if params.use_attractive_pressure else 0.0,
-                g=[float(x) for x in params.gravity])
+                g=[float(x) for x in params.gravity], rho0=float(params.rest_density),
+                wmin=[float(x) for x in params.box_min], wmax=[float(x) for x in params.box_max])
+
+
+def _wall_rho(u):
+    # (315/256) int_u^1 (1 - t^2)^4 dt, zero beyond the support.
+    w = (1.0 - u).clamp(min=0.0)
+    return w ** 5 * (128.0 + 325.0 * u + 345.0 * u ** 2 + 175.0 * u ** 3 + 35.0 * u ** 4) / 256.0
+
+
+def _wall_j(u):
+    # int_u^1 (1 - s)^2 (s^2 - u^2) ds, zero beyond the support.
+    w = (1.0 - u).clamp(min=0.0)
+    return w ** 4 * (1.0 + 4.0 * u) / 30.0
 
 
 def step(x, v, B, mu, k):
@@ -31,7 +45,11 @@ def step(x, v, B, mu, k):
     mask = (d2 < k["h2"]).detach()
     eye = torch.eye(n, dtype=torch.bool)
     t = torch.where(mask, k["h2"] - d2, torch.zeros_like(d2))
-    rho = k["cD"] * (t * t * t).sum(1)
+    wmin = torch.tensor(k["wmin"], dtype=x.dtype)
+    wmax = torch.tensor(k["wmax"], dtype=x.dtype)
+    ul = (x - wmin) / k["h"]
+    uh = (wmax - x) / k["h"]
+    rho = k["cD"] * (t * t * t).sum(1) + k["rho0"] * (_wall_rho(ul) + _wall_rho(uh)).sum(1)
     xr = rho * k["inv_rho0"]
     praw = B * (xr ** 7 - 1.0)
     p = torch.where(praw < 0, praw * k["attr"], praw)
@@ -46,7 +64,8 @@ def step(x, v, B, mu, k):
     cV = mu * k["cP"]
     av = cV * q[:, None] * ((psi * q[None, :])[..., None] * (v[None, :, :] - v[:, None, :])).sum(1)
     g = torch.tensor(k["g"], dtype=x.dtype)
-    a = ap + av + g
+    aw = (90.0 * k["rho0"] / k["h"]) * P[:, None] * (_wall_j(ul) - _wall_j(uh))
+    a = ap + av + g + aw
     v1 = v + k["dt"] * a
     x1 = x + k["dt"] * v1
     lo = torch.tensor(k["lo"], dtype=x.dtype)

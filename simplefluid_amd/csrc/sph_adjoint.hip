@@ -17,6 +17,11 @@
 //                  phi'/d = -t/d^3, psi' = -1; per-particle dL/dB and dL/dmu terms
 //   adj_density:   xbar_i += sum -6 cD t^2 (rhobar_i + rhobar_j) r, then
 //                  scatter both adjoints back to the step's input order.
+// Walls, per axis d (u_lo = (x_d - box_min_d)/h, u_hi = (box_max_d - x_d)/h;
+// forward: rho_i += rho0 wall_rho(u), a_i,d += cW P_i (wall_j(u_lo) - wall_j(u_hi))):
+//   adj_force:     Pbar_i += cW abar_i,d (wall_j(u_lo) - wall_j(u_hi))
+//                  xbar_i,d += cW P_i abar_i,d (wall_j'(u_lo) + wall_j'(u_hi)) / h
+//   adj_density:   xbar_i,d += rhobar_i rho0 (wall_rho'(u_lo) - wall_rho'(u_hi)) / h
 // Each kernel reads neighbour values written by the previous kernel only.
 #include "sph_internal.hpp"
 
@@ -126,6 +131,19 @@ __device__ __forceinline__ double2 adj_force_one(const AdjArgs& a, const SceneCo
         Ay += hr * vj.w * dvy;
         Az += hr * vj.w * dvz;
     });
+    {
+        const float pk[3] = {pi.x, pi.y, pi.z}, ab3[3] = {abi.x, abi.y, abi.z};
+        float gw[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const float ul = (pk[d] - sc.bmin[d]) * sc.inv_h, uh = (sc.wmax[d] - pk[d]) * sc.inv_h;
+            Pb += sc.cW * ab3[d] * (wall_j(ul) - wall_j(uh));
+            gw[d] = ((sc.cW * Pi) * ab3[d]) * (sc.inv_h * (wall_j_du(ul) + wall_j_du(uh)));
+        }
+        gx += gw[0];
+        gy += gw[1];
+        gz += gw[2];
+    }
     // EOS derivatives (clamped Tait, exponent 7).
     const float rho = a.rho[k];
     const float x = rho * sc.inv_rho0;
@@ -199,6 +217,18 @@ __global__ void adj_density_kernel(AdjArgs a) {
             gy += s * ry;
             gz += s * rz;
         });
+        {
+            const float pk[3] = {pi.x, pi.y, pi.z};
+            float gw[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const float ul = (pk[d] - sc.bmin[d]) * sc.inv_h, uh = (sc.wmax[d] - pk[d]) * sc.inv_h;
+                gw[d] = (rbi * sc.rho0) * (sc.inv_h * (wall_rho_du(ul) - wall_rho_du(uh)));
+            }
+            gx += gw[0];
+            gy += gw[1];
+            gz += gw[2];
+        }
         float4 xs = a.xs[k];
         xs.x += gx; xs.y += gy; xs.z += gz;
         const uint32_t src = a.perm[k];

@@ -54,9 +54,13 @@ struct SceneConst {
     float cfl;         // CFL number (0: fixed dt)
     float c0;          // sqrt(7 B / rho0), the EOS sound speed
     float dt_max;      // the parameter dt: the step, or its cap under CFL
-    uint32_t pad[2];
+    float rho0;        // rest density (the walls' half-space fluid)
+    float inv_h;       // 1 / h
+    float cW;          // 2 * 45 rho0 / h: the walls' pressure term
+    float wmax[3];     // box_max (box_min is bmin)
+    uint32_t pad[4];
 };
-static_assert(sizeof(SceneConst) == 160, "SceneConst layout");
+static_assert(sizeof(SceneConst) == 192, "SceneConst layout");
 
 constexpr int kRows = 9;          // 3x3 (dz, dy) rows of three x-adjacent cells
 constexpr int kMaskBits = 32;     // candidates per row tracked in the mask
@@ -272,6 +276,45 @@ __device__ __forceinline__ void wave_max_v2(uint32_t* __restrict__ arr, uint32_t
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v2 = fmaxf(v2, __shfl_xor(v2, o, 64));
     if ((threadIdx.x & 63u) == 0u) atomicMax(arr + idx, __float_as_uint(v2));
+}
+
+// Walls.  Each box face bounds a half-space of fluid at rest density; a
+// particle at distance u h (u < 1) from a face gets that fluid's poly6
+// density rho0 wall_rho(u) and its spiky pressure term, taken at the
+// particle's own p/rho^2, cW (p/rho^2) wall_j(u) along the face's inward
+// normal:
+//   wall_rho(u) = (1-u)^5 (128 + 325u + 345u^2 + 175u^3 + 35u^4) / 256
+//               = (315/256) int_u^1 (1 - t^2)^4 dt          (rho0/2 at u = 0)
+//   wall_j(u)   = (1-u)^4 (1 + 4u) / 30 = int_u^1 (1-s)^2 (s^2 - u^2) ds
+// with the derivatives the reverse pass needs.
+__device__ __forceinline__ float wall_rho(float u) {
+    if (!(u < 1.0f)) return 0.0f;
+    u = fmaxf(u, 0.0f);
+    const float w = 1.0f - u;
+    const float w2 = w * w;
+    const float w5 = (w2 * w2) * w;
+    const float poly = 128.0f + u * (325.0f + u * (345.0f + u * (175.0f + u * 35.0f)));
+    return (w5 * poly) * 0.00390625f;
+}
+__device__ __forceinline__ float wall_j(float u) {
+    if (!(u < 1.0f)) return 0.0f;
+    u = fmaxf(u, 0.0f);
+    const float w = 1.0f - u;
+    const float w2 = w * w;
+    return ((w2 * w2) * (1.0f + 4.0f * u)) * (float)(1.0 / 30.0);
+}
+__device__ __forceinline__ float wall_rho_du(float u) {
+    if (!(u < 1.0f)) return 0.0f;
+    u = fmaxf(u, 0.0f);
+    const float s = 1.0f - u * u;
+    const float s2 = s * s;
+    return -1.23046875f * (s2 * s2);
+}
+__device__ __forceinline__ float wall_j_du(float u) {
+    if (!(u < 1.0f)) return 0.0f;
+    u = fmaxf(u, 0.0f);
+    const float w = 1.0f - u;
+    return -(float)(2.0 / 3.0) * u * ((w * w) * w);
 }
 
 // Tait EOS, exponent 7 by the chain x^2, x^4, (x^4 x^2) x; p < 0 is clamped

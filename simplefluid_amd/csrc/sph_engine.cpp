@@ -100,10 +100,14 @@ int derive_scene(const sph_params& p, SceneConst& sc, std::string& err, int xsub
     sc.cfl = p.cfl_number;
     sc.c0 = (float)std::sqrt(7.0 * (double)p.pressure_stiffness / (double)p.rest_density);
     sc.mu = p.viscosity;
+    sc.rho0 = p.rest_density;
+    sc.inv_h = (float)(1.0 / (double)p.kernel_radius);
+    sc.cW = (float)(90.0 * (double)p.rest_density / (double)p.kernel_radius);
     uint64_t cells = 1;
     for (int k = 0; k < 3; ++k) {
         sc.g[k] = p.gravity[k];
         sc.bmin[k] = p.box_min[k];
+        sc.wmax[k] = p.box_max[k];
         const float ext = p.box_max[k] - p.box_min[k];
         if (!(ext > 2.0f * r)) { err = "box must be wider than 2*particle_radius on every axis"; return SPH_E_INVALID; }
         int G = (int)std::ceil(ext * (k == 0 ? sc.inv_csx : sc.inv_cs));

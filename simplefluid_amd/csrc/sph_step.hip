@@ -70,7 +70,15 @@ __device__ __forceinline__ void density_one(const StepArgs& a, const SceneConst&
         if (rem > 0) density_trip<false>(pi, h2, load4(rx, off), load4(ry, off), load4(rz, off), rem, sum, m);
         a.mask[(size_t)row * a.n + k] = m;
     }
-    const float rho = sc.cD * sum;
+    // The walls' half-space fluid, faces in the order x-, x+, y-, y+, z-, z+.
+    float wr = 0.0f;
+    const float pk[3] = {pi.x, pi.y, pi.z};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        wr += wall_rho((pk[d] - sc.bmin[d]) * sc.inv_h);
+        wr += wall_rho((sc.wmax[d] - pk[d]) * sc.inv_h);
+    }
+    const float rho = sc.cD * sum + sc.rho0 * wr;
     const float p = eos(sc, rho);
     const float ir = 1.0f / rho;
     a.rho[k] = rho;
@@ -137,9 +145,16 @@ __device__ __forceinline__ uint32_t force_one(const StepArgs& a, const SceneCons
             });
     }
     const float cvi = sc.cV * vi.w;
-    const float a0 = (sc.cP * f.px + cvi * f.vx) + sc.g[0];
-    const float a1 = (sc.cP * f.py + cvi * f.vy) + sc.g[1];
-    const float a2 = (sc.cP * f.pz + cvi * f.vz) + sc.g[2];
+    // The walls' pressure term at this particle's own p/rho^2.
+    const float cwi = sc.cW * pi.w;
+    const float pk[3] = {pi.x, pi.y, pi.z};
+    float aw[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+        aw[d] = cwi * (wall_j((pk[d] - sc.bmin[d]) * sc.inv_h) - wall_j((sc.wmax[d] - pk[d]) * sc.inv_h));
+    const float a0 = ((sc.cP * f.px + cvi * f.vx) + sc.g[0]) + aw[0];
+    const float a1 = ((sc.cP * f.py + cvi * f.vy) + sc.g[1]) + aw[1];
+    const float a2 = ((sc.cP * f.pz + cvi * f.vz) + sc.g[2]) + aw[2];
     if (a.acc) a.acc[k] = make_float4(a0, a1, a2, 0.0f);
     float v[3] = {vi.x + sc.dt * a0, vi.y + sc.dt * a1, vi.z + sc.dt * a2};
     float p[3] = {pi.x + sc.dt * v[0], pi.y + sc.dt * v[1], pi.z + sc.dt * v[2]};

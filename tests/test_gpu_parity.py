@@ -622,6 +622,65 @@ def test_run_frame_matches_reference_loop(E):
     assert s == n and t == ft
 
 
+def _session_to_stop_time(E, p, pos, every=15):
+    """Frames until the float sim time reaches stopTime (Source/Simulator.cpp:44-51,
+    m_SimTime is a float); returns frame count, the distinct step counts per
+    frame and (v^2/2 + g.y) per unit mass every `every` frames, after checking
+    that the state is finite and inside the clamp bounds."""
+    e = make_engine(E, p, pos)
+    g = -float(p.gravity[1])
+    r = np.float32(p.particle_radius)
+    lo = np.array(p.box_min[:], np.float32) + r
+    hi = np.array(p.box_max[:], np.float32) - r
+    sim_time, frames, steps, energies = np.float32(0), 0, set(), []
+    while sim_time < np.float32(p.stop_time):
+        t, k = e.run_frame()
+        sim_time = np.float32(sim_time + np.float32(t))
+        frames += 1
+        steps.add(k)
+        if frames % every == 0:
+            x, v = e.get_particles(), e.get_velocity()
+            assert np.isfinite(x).all() and np.isfinite(v).all()
+            assert (x >= lo).all() and (x <= hi).all()
+            ke = 0.5 * float(np.mean((v.astype(np.float64) ** 2).sum(1)))
+            energies.append(ke + g * float(np.mean(x[:, 1].astype(np.float64) - lo[1])))
+    e.close()
+    return frames, steps, energies
+
+
+def test_reference_double_dambreak_runs_to_stop_time(E):
+    """The reference's default session: DoubleDambreak at res 24 with the GUI
+    defaults runs its 5 s in 150 frames of equal step counts, every state
+    finite and inside the clamp bounds, and v^2/2 + g.y per unit mass stays
+    bounded: below twice its value at rest, and lower at the end than half a
+    second in.  (The collapse itself adds energy at this stiffness: poly6
+    density with a spiky pressure gradient is not an energy-conserving
+    pairing; the walls' half-space terms keep it from growing afterwards.)"""
+    p = E.default_params(24)
+    pos = E.scene_generate(E.SCENE_DOUBLE_DAMBREAK, 24)
+    g = -float(p.gravity[1])
+    e0 = g * float(np.mean(pos[:, 1].astype(np.float64) - (np.float32(p.box_min[1]) + np.float32(p.particle_radius))))
+    frames, steps, energies = _session_to_stop_time(E, p, pos)
+    say(f"[stop time, B 50000] {frames} frames of {sorted(steps)} steps, energy per unit mass {e0:.3f} -> "
+        f"{[round(v, 2) for v in energies]}")
+    assert frames == 150 and len(steps) == 1
+    assert max(energies) < 2.0 * e0 and energies[-1] < energies[0]
+
+
+def test_softer_dambreak_session_loses_energy(E):
+    """The same session at stiffness 5000 only loses energy (viscosity and
+    the inelastic walls): v^2/2 + g.y per unit mass stays below its value at
+    rest and ends lower than it was after the first half second."""
+    p = E.default_params(24, pressure_stiffness=5000.0)
+    pos = E.scene_generate(E.SCENE_DOUBLE_DAMBREAK, 24)
+    g = -float(p.gravity[1])
+    e0 = g * float(np.mean(pos[:, 1].astype(np.float64) - (np.float32(p.box_min[1]) + np.float32(p.particle_radius))))
+    frames, _, energies = _session_to_stop_time(E, p, pos)
+    say(f"[stop time, B 5000] energy per unit mass {e0:.3f} -> {[round(v, 3) for v in energies]}")
+    assert frames == 150
+    assert max(energies) < e0 and energies[-1] < energies[0]
+
+
 def test_qt_shaped_solver(E):
     p = E.default_params(16)
     s = E.SPHSolver(p)
@@ -655,9 +714,13 @@ def test_particle_count_change_recaptures_graph(E):
 
 
 def test_single_particle_and_empty_input(E):
-    """One particle has only itself as neighbour: density m W(0), p clamped
-    to 0, and it falls under gravity along the float32 symplectic-Euler
-    closed form, onto the floor clamp.  Zero particles cannot be made ready."""
+    """One particle has only itself as neighbour: density m W(0) plus the
+    floor's half-space share (it starts 0.6 h above the floor; the value is
+    the float32 oracle's, same operations), p clamped to 0 (below rest
+    density), so no pressure of its own or of the wall acts and it falls
+    under gravity along the float32 symplectic-Euler closed form, onto the
+    floor clamp.  Zero particles cannot be made ready."""
+    import oracle
     p = E.make_params()
     x0 = np.array([[0.5, 0.012, 0.5]], np.float32)
     e = make_engine(E, p, x0)
@@ -671,7 +734,12 @@ def test_single_particle_and_empty_input(E):
     for k in range(200):
         e.advance_steps(1)
         if k == 0:
-            assert e.get_density()[0] == np.float32(c["cD"]) * np.float32(np.float32(t * t) * t)
+            o = oracle.Oracle(to_oracle(p, c["xsub"]))
+            o.set_state(x0.astype(np.float64))
+            o.step()
+            rho_o = np.float32(o.step_debug()["rho"][0])
+            assert rho_o > np.float32(c["cD"]) * np.float32(np.float32(t * t) * t)
+            assert e.get_density()[0] == rho_o
         v = v + dt * g                             # a = (0 + 0) + g: no pressure, no viscosity
         x = x + dt * v
         if x[0, 1] < lo:

@@ -219,6 +219,66 @@ def test_isolated_particle_density_and_free_fall(oracle_lib):
         assert x[0, 0] == pytest.approx(0.5 + 0.1 * dt * s, rel=1e-9)
 
 
+def _half_space_sum(h, d, f):
+    """sum f(dz, r2) s^3 over the midpoints of s-cubes of the half-space z < 0
+    within h of (0, 0, d) (dz = d - z), at s = h/60 and h/120, combined by
+    Richardson extrapolation (the midpoint sum's error is O(s^2))."""
+    def grid_sum(n):
+        s = h / n
+        ax = (np.arange(-n, n) + 0.5) * s
+        X, Y = np.meshgrid(ax, ax, indexing="ij")
+        tot = 0.0
+        for z in -(np.arange(0, n) + 0.5) * s:
+            dz = d - z
+            r2 = X ** 2 + Y ** 2 + dz * dz
+            m = r2 < h * h
+            tot += f(dz, r2[m]).sum()
+        return tot * s ** 3
+    return (4.0 * grid_sum(120) - grid_sum(60)) / 3.0
+
+
+@pytest.mark.parametrize("u", [0.0, 0.25, 0.5, 0.8])
+def test_wall_density_is_the_half_space_integral(oracle_lib, u):
+    """A lone particle at u h above the floor: its density beyond the self term
+    is rest fluid filling the half-space below the floor, summed with poly6
+    over a fine grid of cubes (independent of the closed form)."""
+    import oracle
+    h, rho0 = 0.02, 1000.0
+    o = oracle.Oracle(oracle.make_params(kernel_radius=h, particle_radius=h / 4, rest_density=rho0), fp64=True)
+    o.set_state(np.array([[0.5, 0.5, u * h]]))
+    o.step()
+    c = o.constants()
+    wall = o.step_debug()["rho"][0] - c["cD"] * c["h2"] ** 3
+    brute = rho0 * _half_space_sum(h, u * h, lambda dz, r2: 315 / (64 * math.pi * h ** 9) * (h * h - r2) ** 3)
+    assert wall == pytest.approx(brute, rel=1e-4)
+    if u == 0.0:
+        assert wall == pytest.approx(rho0 / 2, rel=1e-12)
+
+
+@pytest.mark.parametrize("u", [0.25, 0.5, 0.8])
+def test_wall_pressure_force_is_the_half_space_sum(oracle_lib, u):
+    """A lone particle with negative pressure (attractive option at ratio 1,
+    no gravity or viscosity) near the floor: its acceleration is the spiky
+    pressure term of the rest fluid below the floor taken at its own p/rho^2,
+    summed over a fine grid of cubes, and it points at the floor."""
+    import oracle
+    h, rho0, B = 0.02, 1000.0, 1000.0
+    prm = oracle.make_params(kernel_radius=h, particle_radius=h / 4, rest_density=rho0, pressure_stiffness=B,
+                             viscosity=0.0, gravity=(0.0, 0.0, 0.0), use_attractive_pressure=1,
+                             attractive_pressure_ratio=1.0)
+    o = oracle.Oracle(prm, fp64=True)
+    o.set_state(np.array([[0.5, 0.5, u * h]]))
+    o.step()
+    d = o.step_debug()
+    rho = d["rho"][0]
+    q = B * ((rho / rho0) ** 7 - 1.0) / rho ** 2
+    spiky = _half_space_sum(h, u * h, lambda dz, r2: (h - np.sqrt(r2)) ** 2 * dz / np.sqrt(r2))
+    brute = 2 * q * rho0 * 45 / (math.pi * h ** 6) * spiky
+    assert q < 0 and d["acc"][0, 2] < 0
+    assert d["acc"][0, 2] == pytest.approx(brute, rel=1e-4)
+    assert abs(d["acc"][0, 0]) < 1e-12 and abs(d["acc"][0, 1]) < 1e-12
+
+
 def test_poly6_normalisation():
     """m sum W over a fine lattice -> rho0 * (integral of W = 1): checks 315/(64 pi h^9)."""
     h = 0.02
