@@ -15,8 +15,11 @@ threads, and one thread).  Extras: `batched` (256-scene ensemble, one launch
 per pass, vs the same scenes launched one engine at a time), `adjoint`
 (rollout reverse pass vs forward wall-clock, 16 x 32,768 particles, 200
 steps, checkpoint every 20), `dambreak_65k` (the 65,536-particle dam break,
-1000 steps) and `sphere_4m` (the 4.19M-particle sphere drop over a resting
-layer, r scaled to 0.0015).
+1000 steps), `sphere_4m` (the 4.19M-particle sphere drop over a resting
+layer, r scaled to 0.0015) and `reference_session` (frames per second of the
+reference application's default session: DoubleDambreak at resolution 24,
+GUI defaults, 334 steps per 1/30 s frame; `cpu_baseline` times one of its
+frames on the oracle).
 """
 from __future__ import annotations
 
@@ -223,12 +226,22 @@ def cpu_baseline(cfg, budget_s):
     v1, k1 = run(cfg, 1, budget_s / 3.0, 3, 1)
     small = configs.dambreak_65k()
     vs, ks = run(small, threads, budget_s / 3.0, 300, 10)
+    # One frame of the reference's default session (334 steps) after one warm-up step.
+    from simplefluid_amd import engine
+    rp, rpos = reference_session(engine)
+    o = oracle.Oracle(oracle.from_params(rp, 4), threads=threads)
+    o.set_state(rpos.astype(np.float64))
+    o.step(1)
+    frame_s = o.step(334)
     return {"value": round(v, 1), "unit": "particle-steps/s", "cores": threads, "kind": "port",
             "sample": f"oracle C++ FP32 step (OpenMP, {threads} threads), {n} particles of the same workload: "
                       f"median of 3 chunks of {k} steps after 10 warm-up steps",
             "single_thread": {"value": round(v1, 1), "sample": f"median of 3 chunks of {k1} steps"},
             "dambreak_65k": {"value": round(vs, 1), "threads": threads,
                              "sample": f"median of 3 chunks of {ks} of its 1000 steps after 11 warm-up steps"},
+            "reference_session": {"ms_per_frame": round(frame_s * 1e3, 1), "threads": threads,
+                                  "sample": "one 334-step frame of DoubleDambreak at resolution 24 "
+                                            "(GUI defaults) after one warm-up step"},
             "host": host_info()}
 
 
@@ -327,8 +340,45 @@ def sphere_4m_leg(args):
 
 
 def small_and_large(args):
-    """The 65k dam break over its full 1000 steps, and the 4.4M sphere drop."""
-    return {"dambreak_65k": dambreak_65k_leg(args), "sphere_4m": sphere_4m_leg(args)}
+    """The 65k dam break over its full 1000 steps, the 4.4M sphere drop, and
+    the reference application's default session."""
+    return {"dambreak_65k": dambreak_65k_leg(args), "sphere_4m": sphere_4m_leg(args),
+            "reference_session": reference_session_leg(args)}
+
+
+REF_FRAMES = 30
+
+
+def reference_session(engine):
+    """The reference's default session (Source/Simulator.cpp:33-63): the
+    DoubleDambreak scene at resolution 24 with the GUI defaults."""
+    return engine.default_params(24), engine.scene_generate(engine.SCENE_DOUBLE_DAMBREAK, 24)
+
+
+def reference_session_leg(args):
+    """Frames per second of the reference's default session: each frame is
+    the steps whose float sum reaches 1/30 s (334 of 1e-4 s), run on the
+    device; 30 frames after 3 untimed ones."""
+    from simplefluid_amd import engine
+    p, pos = reference_session(engine)
+    e = engine.Engine(p)
+    e.set_particles(pos)
+    e.make_ready()
+    for _ in range(3):
+        e.run_frame()
+    e.synchronize()
+    t0 = time.perf_counter()
+    steps = 0
+    for _ in range(REF_FRAMES):
+        steps += e.run_frame()[1]
+    e.synchronize()
+    t = time.perf_counter() - t0
+    e.close()
+    out = {"scene": "DoubleDambreak", "resolution": 24, "particles": int(pos.shape[0]), "frames": REF_FRAMES,
+           "steps_per_frame": steps // REF_FRAMES, "ms_per_frame": round(t / REF_FRAMES * 1e3, 3),
+           "frames_per_s": round(REF_FRAMES / t, 1)}
+    log("reference_session", out)
+    return out
 
 
 def rollout_targets(rc):
@@ -422,7 +472,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
-    ap.add_argument("--only", choices=["plain", "batched", "adjoint", "dambreak65k", "sphere4m"], default=None,
+    ap.add_argument("--only", choices=["plain", "batched", "adjoint", "dambreak65k", "sphere4m", "reference"],
+                    default=None,
                     help="run one leg alone (profiling)")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--checkpoint-every", type=int, default=0,
@@ -453,6 +504,9 @@ def main():
         return
     if args.only == "sphere4m":
         print(json.dumps({"sphere_4m": sphere_4m_leg(args)}), flush=True)
+        return
+    if args.only == "reference":
+        print(json.dumps({"reference_session": reference_session_leg(args)}), flush=True)
         return
     value, step_ms, roof, extra, cfg = plain(args)
     out = {"metric": METRIC, "value": round(value, 1), "unit": "particle-steps/s", "n_gpus": 1,
