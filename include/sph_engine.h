@@ -70,8 +70,11 @@ typedef struct sph_params {
     float attractive_pressure_ratio;
     float gravity[3];
     float dt;                        /* fixed step returned by advance_frame      */
-    float box_min[3];
-    float box_max[3];
+    float box_min[3];                /* box faces: each bounds a half-space of    */
+    float box_max[3];                /* rest fluid (its poly6 density and spiky
+                                        pressure reach particles within h of the
+                                        face); positions are also clamped to
+                                        [box_min + r, box_max - r]               */
     float stop_time;                 /* driver only (Source/Simulator.cpp:44)     */
     int   num_threads;               /* accepted for parity, unused on the GPU    */
     float cfl_number;                /* 0: every step is dt.  > 0: a step is
