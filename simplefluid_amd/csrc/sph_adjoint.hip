@@ -164,8 +164,9 @@ __device__ __forceinline__ double2 adj_force_one(const AdjArgs& a, const SceneCo
     return make_double2((double)bpart, (double)mpart);
 }
 
-// Eight waves per SIMD (64 VGPRs): 4 % faster than the six its registers
-// would otherwise allow.
+// Eight waves per SIMD (64 VGPRs, a 48-byte spill): the backward pass is
+// 3 % faster than without the bound (47.0 against 48.3 ms, 16 x 32k
+// particles x 200 steps).
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) void adj_force_kernel(AdjArgs a) {
     uint32_t t0, t1;
     tile_span(a.ntiles, t0, t1);
