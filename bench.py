@@ -364,6 +364,11 @@ def reference_session_leg(args):
     e = engine.Engine(p)
     e.set_particles(pos)
     e.make_ready()
+    t_tune = 0.0
+    if not args.no_autotune:
+        t0 = time.perf_counter()
+        e.autotune()
+        t_tune = time.perf_counter() - t0
     for _ in range(3):
         e.run_frame()
     e.synchronize()
@@ -373,10 +378,12 @@ def reference_session_leg(args):
         steps += e.run_frame()[1]
     e.synchronize()
     t = time.perf_counter() - t0
+    e_tuning = e.get_tuning()
     e.close()
     out = {"scene": "DoubleDambreak", "resolution": 24, "particles": int(pos.shape[0]), "frames": REF_FRAMES,
            "steps_per_frame": steps // REF_FRAMES, "ms_per_frame": round(t / REF_FRAMES * 1e3, 3),
-           "frames_per_s": round(REF_FRAMES / t, 1)}
+           "frames_per_s": round(REF_FRAMES / t, 1), "tuning": e_tuning,
+           "autotune_ms": round(t_tune * 1e3, 1)}
     log("reference_session", out)
     return out
 

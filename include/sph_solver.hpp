@@ -143,7 +143,9 @@ public:
     QtSPHSolver(const QtSPHSolver&) = delete;
     QtSPHSolver& operator=(const QtSPHSolver&) = delete;
 
-    /* Source/Simulator.cpp:42: parameters and the host arrays go to the device. */
+    /* Source/Simulator.cpp:42: parameters and the host arrays go to the device;
+     * then block sizes are picked for this size (sph_autotune: about 10 ms the
+     * first time, cached per process; no result depends on them). */
     void makeReady() {
         const sph_params p = params_->to_c();
         check(sph_set_params(h_, &p));
@@ -153,6 +155,7 @@ public:
         check(sph_set_particles(h_, reinterpret_cast<const float*>(particles_.data()), v,
                                 (uint32_t)particles_.size()));
         check(sph_make_ready(h_));
+        check(sph_autotune(h_));
         velocities_.resize(particles_.size(), Vec3{0.f, 0.f, 0.f});
         host_stale_ = false;
     }

@@ -503,7 +503,10 @@ class SPHSolver:
         self.engine.set_particles(xyz, vel)
 
     def makeReady(self):
+        """Uploads and builds the grid, then picks block sizes for this
+        handle's size (sph_autotune; cached per process, results unchanged)."""
         self.engine.make_ready()
+        self.engine.autotune()
 
     def advanceFrame(self) -> float:
         return self.engine.advance_frame()
