@@ -379,10 +379,16 @@ def reference_session_leg(args):
     e.synchronize()
     t = time.perf_counter() - t0
     e_tuning = e.get_tuning()
+    # Per-pass times (HIP events on the engine's stream) over 100 more steps.
+    e.profile_enable(True)
+    e.advance_steps(100)
+    ms, nsteps = e.profile_read()
+    e.profile_enable(False)
+    pass_us = {k: round(v / max(nsteps, 1) * 1e3, 1) for k, v in ms.items()}
     e.close()
     out = {"scene": "DoubleDambreak", "resolution": 24, "particles": int(pos.shape[0]), "frames": REF_FRAMES,
            "steps_per_frame": steps // REF_FRAMES, "ms_per_frame": round(t / REF_FRAMES * 1e3, 3),
-           "frames_per_s": round(REF_FRAMES / t, 1), "tuning": e_tuning,
+           "frames_per_s": round(REF_FRAMES / t, 1), "pass_us": pass_us, "tuning": e_tuning,
            "autotune_ms": round(t_tune * 1e3, 1)}
     log("reference_session", out)
     return out
