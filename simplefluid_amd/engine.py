@@ -504,9 +504,13 @@ class SPHSolver:
 
     def makeReady(self):
         """Uploads and builds the grid, then picks block sizes for this
-        handle's size (sph_autotune; cached per process, results unchanged)."""
+        handle's size (sph_autotune; cached per process, results unchanged).
+        A failed autotune leaves the state as it was and keeps the defaults."""
         self.engine.make_ready()
-        self.engine.autotune()
+        try:
+            self.engine.autotune()
+        except SPHError:
+            pass
 
     def advanceFrame(self) -> float:
         return self.engine.advance_frame()
