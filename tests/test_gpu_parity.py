@@ -195,6 +195,32 @@ def test_double_dambreak_1m_long_run_stays_physical(E):
     e.close()
 
 
+def test_sphere_drop_4m_full_run_stays_physical(E):
+    """The 4.4M-particle sphere drop over its full 300 steps (the largest
+    plain workload, x subdivision 4, autotuned): every particle finite and
+    inside the clamp bounds, ids a permutation, the sphere's particles
+    falling towards the resting layer."""
+    from simplefluid_amd import configs
+    cfg = configs.sphere_drop_4m()
+    p = cfg["params"]
+    e = make_engine(E, p, cfg["pos"])
+    e.autotune()
+    ns = cfg["n_sphere"]
+    ys0 = float(e.get_particles()[:ns, 1].mean())
+    e.advance_steps(cfg["steps"])
+    x, v = e.get_particles(), e.get_velocity()
+    _, _, ids = e.debug_get_state()
+    assert np.isfinite(x).all() and np.isfinite(v).all()
+    lo = np.array(p.box_min[:], np.float32) + np.float32(p.particle_radius)
+    hi = np.array(p.box_max[:], np.float32) - np.float32(p.particle_radius)
+    assert (x >= lo).all() and (x <= hi).all()
+    assert np.array_equal(np.sort(ids), np.arange(len(ids), dtype=np.uint32))
+    ys1 = float(x[:ns, 1].mean())
+    say(f"[sphere 4m] {len(x)} particles, sphere mean height {ys0:.4f} -> {ys1:.4f} after {cfg['steps']} steps")
+    assert ys1 < ys0
+    e.close()
+
+
 def test_ensemble_256_full_run_stays_physical(E):
     """The 256-scene ensemble over its full 1000 steps in one handle (5.7M
     particles): every scene's particles finite, inside that scene's clamp
