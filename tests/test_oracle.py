@@ -421,6 +421,27 @@ def test_golden_fixture_tiny(oracle_lib):
     assert np.allclose(d["acc"], g["acc"], rtol=1e-10, atol=1e-12)
 
 
+def test_golden_fixture_wall(oracle_lib):
+    """Committed FP64 vectors of a step in the box corner
+    (tests/golden/wall_step.npz, written by tests/golden/make_golden.py once
+    the wall terms matched their half-space sums) still match the oracle."""
+    import oracle
+    g = np.load(os.path.join(HERE, "golden", "wall_step.npz"))
+    o = oracle.Oracle(oracle.make_params(pressure_stiffness=float(g["B"]), viscosity=float(g["mu"])), fp64=True)
+    o.set_state(g["pos"], g["vel"])
+    o.step()
+    d = o.step_debug()
+    assert np.array_equal(d["perm"], g["perm"])
+    assert np.allclose(d["rho"], g["rho"], rtol=1e-12)
+    assert np.allclose(d["acc"], g["acc"], rtol=1e-10, atol=1e-12)
+    # The walls act: the floor's half-space makes up the floor layer's missing
+    # neighbours (its mean density is the rest density to 1 %), and the layer
+    # is held up against gravity.
+    floor = g["pos"][d["perm"], 1] < 0.0075
+    assert abs(g["rho"][floor].mean() - 1000.0) < 10.0
+    assert g["acc"][floor, 1].mean() > -9.81
+
+
 # ------------------------------------------------------------- adjoint ----
 
 def _rollout_case(n3=(6, 6, 6), steps=8, seed=0):
