@@ -19,8 +19,9 @@
 //                  scatter both adjoints back to the step's input order.
 // Walls, per axis d (u_lo = (x_d - box_min_d)/h, u_hi = (box_max_d - x_d)/h;
 // forward: rho_i += rho0 wall_rho(u), a_i,d += cW P_i (wall_j(u_lo) - wall_j(u_hi))):
-//   adj_force:     Pbar_i += cW abar_i,d (wall_j(u_lo) - wall_j(u_hi))
-//                  xbar_i,d += cW P_i abar_i,d (wall_j'(u_lo) + wall_j'(u_hi)) / h
+//   adj_integrate: xbar_i,d += cW P_i abar_i,d (wall_j'(u_lo) + wall_j'(u_hi)) / h,
+//                  and cW abar_i,d (wall_j(u_lo) - wall_j(u_hi)) in abar.w, where
+//   adj_force      starts Pbar_i from it (the term involves particle i alone)
 //   adj_density:   xbar_i,d += rhobar_i rho0 (wall_rho'(u_lo) - wall_rho'(u_hi)) / h
 // Each kernel reads neighbour values written by the previous kernel only.
 #include "sph_internal.hpp"
@@ -91,9 +92,18 @@ __global__ void adj_integrate_kernel(AdjArgs a) {
             vo[d] = vb1;
             ab[d] = sc.dt * vb1;
         }
+        // Walls: their pressure term depends on this particle alone, so its
+        // reverse (Pbar share in abar.w, xbar here) is elementwise.
+        float pw = 0.0f;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const float ul = (xa[d] - sc.bmin[d]) * sc.inv_h, uh = (sc.wmax[d] - xa[d]) * sc.inv_h;
+            pw += sc.cW * ab[d] * (wall_j(ul) - wall_j(uh));
+            xo[d] += ((sc.cW * x0.w) * ab[d]) * (sc.inv_h * (wall_j_du(ul) + wall_j_du(uh)));
+        }
         a.xs[k] = make_float4(xo[0], xo[1], xo[2], 0.0f);
         a.vs[k] = make_float4(vo[0], vo[1], vo[2], 0.0f);
-        a.abar[k] = make_float4(ab[0], ab[1], ab[2], 0.0f);
+        a.abar[k] = make_float4(ab[0], ab[1], ab[2], pw);
     }
 }
 
@@ -101,7 +111,7 @@ __global__ void adj_integrate_kernel(AdjArgs a) {
 __device__ __forceinline__ double2 adj_force_one(const AdjArgs& a, const SceneConst& sc, uint32_t k) {
     const float4 pi = a.pos_s[k], vi = a.vel_s[k], abi = a.abar[k];
     const float Pi = pi.w, qi = vi.w;  // p/rho^2 and 1/rho, stored by the density pass
-    float gx = 0, gy = 0, gz = 0, hx = 0, hy = 0, hz = 0, Pb = 0, qb = 0, Ax = 0, Ay = 0, Az = 0;
+    float gx = 0, gy = 0, gz = 0, hx = 0, hy = 0, hz = 0, Pb = abi.w, qb = 0, Ax = 0, Ay = 0, Az = 0;
     for_each_neighbor(a, sc, k, pi, [&](uint32_t j) {
         const float4 pj = a.pos_s[j], vj = a.vel_s[j], abj = a.abar[j];
         const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
@@ -131,19 +141,6 @@ __device__ __forceinline__ double2 adj_force_one(const AdjArgs& a, const SceneCo
         Ay += hr * vj.w * dvy;
         Az += hr * vj.w * dvz;
     });
-    {
-        const float pk[3] = {pi.x, pi.y, pi.z}, ab3[3] = {abi.x, abi.y, abi.z};
-        float gw[3];
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            const float ul = (pk[d] - sc.bmin[d]) * sc.inv_h, uh = (sc.wmax[d] - pk[d]) * sc.inv_h;
-            Pb += sc.cW * ab3[d] * (wall_j(ul) - wall_j(uh));
-            gw[d] = ((sc.cW * Pi) * ab3[d]) * (sc.inv_h * (wall_j_du(ul) + wall_j_du(uh)));
-        }
-        gx += gw[0];
-        gy += gw[1];
-        gz += gw[2];
-    }
     // EOS derivatives (clamped Tait, exponent 7).
     const float rho = a.rho[k];
     const float x = rho * sc.inv_rho0;
@@ -164,9 +161,9 @@ __device__ __forceinline__ double2 adj_force_one(const AdjArgs& a, const SceneCo
     return make_double2((double)bpart, (double)mpart);
 }
 
-// Eight waves per SIMD (64 VGPRs, a 48-byte spill): the backward pass is
+// Eight waves per SIMD (64 VGPRs, a 36-byte spill): the backward pass was
 // 3 % faster than without the bound (47.0 against 48.3 ms, 16 x 32k
-// particles x 200 steps).
+// particles x 200 steps, with the walls' reverse still in this kernel).
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) void adj_force_kernel(AdjArgs a) {
     uint32_t t0, t1;
     tile_span(a.ntiles, t0, t1);
