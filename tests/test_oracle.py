@@ -301,21 +301,179 @@ def test_spiky_normalisation():
     assert np.allclose(np.gradient(W, r)[1:-1], dW[1:-1], rtol=1e-3, atol=1e-3 * abs(dW).max())
 
 
-def test_eos_closed_form(oracle_lib):
-    """rho = rho0 gives p = 0; compressed gives B((rho/rho0)^7 - 1); rarefied is clamped."""
+# ------------------------------ pair terms against the published formulas ----
+#
+# These pins take nothing from the oracle but its outputs: the constants are
+# written here from the papers (Mueller, Charypar and Gross 2003, "Particle-
+# based fluid simulation for interactive applications": W_poly6 = 315/(64 pi
+# h^9)(h^2 - r^2)^3, grad W_spiky = -45/(pi h^6)(h - r)^2 r_hat, lap W_visc =
+# 45/(pi h^6)(h - r); Becker and Teschner 2007: p = B((rho/rho0)^7 - 1)) and
+# from the lattice-cell mass m = rho0 (2r)^3.  A wrong normalisation (15 for
+# 45, h^5 for h^6), a dropped 1/rho_i or 1/rho_j, or another mass rule moves
+# the oracle's density or acceleration far outside the tolerances.
+
+def _f32(v):
+    return float(np.float32(v))   # the oracle's parameters are float
+
+
+def _textbook_step(pos, vel, h, r, rho0, B, mu, g, dt, attractive=False, ratio=0.0):
+    """One step of the formulation, FP64, all pairs, for particles farther
+    than h from every box face (no wall terms): density with the self term,
+    Tait EOS with the clamp (or the attractive ratio), symmetric pressure
+    force, Laplacian viscosity, gravity, symplectic Euler."""
+    m = rho0 * (2.0 * r) ** 3
+    n = len(pos)
+    rij = pos[:, None, :] - pos[None, :, :]
+    d = np.linalg.norm(rij, axis=2)
+    near = d < h
+    W = np.where(near, 315.0 / (64.0 * math.pi * h ** 9) * (h * h - d * d) ** 3, 0.0)
+    rho = m * W.sum(1)
+    p_raw = B * ((rho / rho0) ** 7 - 1.0)
+    p = np.where(p_raw < 0.0, p_raw * ratio if attractive else 0.0, p_raw)
+    pair = near & ~np.eye(n, dtype=bool)
+    ds = np.where(pair, d, 1.0)
+    gradW = np.where(pair[..., None], (-45.0 / (math.pi * h ** 6) * (h - ds) ** 2 / ds)[..., None] * rij, 0.0)
+    a_p = -(m * (p[:, None] / rho[:, None] ** 2 + p[None, :] / rho[None, :] ** 2))[..., None] * gradW
+    lapW = np.where(pair, 45.0 / (math.pi * h ** 6) * (h - ds), 0.0)
+    a_v = (mu / rho)[:, None, None] * (m * lapW / rho[None, :])[..., None] * (vel[None, :, :] - vel[:, None, :])
+    a = a_p.sum(1) + a_v.sum(1) + np.asarray(g)
+    v1 = vel + dt * a
+    x1 = pos + dt * v1
+    return dict(rho=rho, p=p, p_raw=p_raw, a=a, a_p=a_p.sum(1), x1=x1, v1=v1, m=m)
+
+
+def _oracle_step(pos, vel, **kw):
+    """One FP64 oracle step; density and acceleration back in input order."""
     import oracle
-    pos = lattice((6, 6, 6))
-    for B in (50.0, 50000.0):
-        o = oracle.Oracle(oracle.make_params(pressure_stiffness=B), fp64=True)
-        o.set_state(pos)
-        o.step()
-        c = o.constants()
-        rho = o.step_debug()["rho"]
-        # reconstruct pressure force scale from the force pass indirectly: interior
-        # particles have rho > rho0 here, boundary particles rho < rho0
-        assert rho.max() > 1000.0 > rho.min()
-        x = rho.max() / 1000.0
-        assert B * (x ** 7 - 1) > 0
+    o = oracle.Oracle(oracle.make_params(**kw), fp64=True)
+    o.set_state(pos, vel)
+    o.step()
+    d = o.step_debug()
+    rho = np.empty(len(pos)); acc = np.empty((len(pos), 3))
+    rho[d["perm"]] = d["rho"]
+    acc[d["perm"]] = d["acc"]
+    x1, v1 = o.state_by_id()
+    return rho, acc, x1, v1
+
+
+def _pair_case(q, direction, h=0.02, r_over_h=0.4, seed=0):
+    """Two particles q h apart along `direction`, around the box centre."""
+    u = np.asarray(direction, float)
+    u /= np.linalg.norm(u)
+    h = _f32(h)
+    c = np.array([0.5, 0.5, 0.5])
+    pos = np.stack([c - 0.5 * q * h * u, c + 0.5 * q * h * u])
+    vel = np.random.default_rng(seed).normal(0.0, 0.3, (2, 3))
+    return pos, vel, h, _f32(r_over_h * h)
+
+
+def _check_against_textbook(pos, vel, h, r, B, mu, rho0=1000.0, g=(0.0, -9.81, 0.0), dt=1e-4,
+                            attractive=False, ratio=0.0):
+    want = _textbook_step(pos, vel, h, r, _f32(rho0), _f32(B), _f32(mu), [_f32(x) for x in g], _f32(dt),
+                          attractive, _f32(ratio))
+    rho, acc, x1, v1 = _oracle_step(pos, vel, kernel_radius=h, particle_radius=r, rest_density=rho0,
+                                    pressure_stiffness=B, viscosity=mu, gravity=g, dt=dt,
+                                    use_attractive_pressure=int(attractive), attractive_pressure_ratio=ratio)
+    np.testing.assert_allclose(rho, want["rho"], rtol=1e-12)
+    scale = np.abs(want["a"]).max()
+    np.testing.assert_allclose(acc, want["a"], rtol=0, atol=1e-11 * scale)
+    np.testing.assert_allclose(v1, want["v1"], rtol=0, atol=1e-11 * max(1.0, np.abs(want["v1"]).max()))
+    np.testing.assert_allclose(x1, want["x1"], rtol=0, atol=1e-13)
+    return want
+
+
+def test_two_particle_step_compressed_branch(oracle_lib):
+    """r = 0.4 h and 0.38 h apart: rho/rho0 ~ 1.3, p > 0; pressure and
+    viscosity terms, gravity and the step all as published."""
+    pos, vel, h, r = _pair_case(0.38, (1.0, 2.0, 2.0))
+    w = _check_against_textbook(pos, vel, h, r, B=1000.0, mu=0.1)
+    assert (w["rho"] > 1.2 * 1000.0).all() and (w["p"] > 0).all()
+    # The pair repels: particle 0's pressure acceleration points away from 1.
+    assert np.dot(w["a_p"][0], pos[0] - pos[1]) > 0
+
+
+def test_two_particle_step_clamped_branch(oracle_lib):
+    """0.9 h apart: rarefied, p < 0 is clamped to 0, so only viscosity and
+    gravity act."""
+    pos, vel, h, r = _pair_case(0.9, (0.0, 1.0, 0.0), seed=1)
+    w = _check_against_textbook(pos, vel, h, r, B=1000.0, mu=0.2)
+    assert (w["p_raw"] < 0).all() and (w["p"] == 0).all()
+    assert np.abs(w["a_p"]).max() == 0.0
+
+
+def test_two_particle_step_attractive_branch(oracle_lib):
+    """Attractive pressure on: p < 0 is scaled by the ratio and the pair
+    attracts (Source/Controller.cpp:61 exposes the option)."""
+    pos, vel, h, r = _pair_case(0.7, (3.0, -1.0, 0.5), seed=2)
+    w = _check_against_textbook(pos, vel, h, r, B=2000.0, mu=0.05, attractive=True, ratio=0.3)
+    assert (w["p"] < 0).all()
+    assert np.dot(w["a_p"][0], pos[0] - pos[1]) < 0
+
+
+def test_three_particle_step_unequal_densities(oracle_lib):
+    """Three particles at unequal spacings, so rho_i, p_i differ per particle:
+    the p_i/rho_i^2 + p_j/rho_j^2 pairing and the 1/rho_i, 1/rho_j factors of
+    the viscosity are told apart."""
+    h = _f32(0.02)
+    c = np.array([0.5, 0.5, 0.5])
+    pos = c + h * np.array([[0.0, 0.0, 0.0], [0.33, 0.1, 0.0], [-0.1, 0.55, 0.3]])
+    vel = np.random.default_rng(3).normal(0.0, 0.5, (3, 3))
+    w = _check_against_textbook(pos, vel, h, _f32(0.42 * h), B=500.0, mu=0.3)
+    rho = w["rho"]
+    assert len(np.unique(np.round(rho, 6))) == 3 and (w["p"] > 0).any()
+
+
+def test_eos_closed_form(oracle_lib):
+    """The oracle's pressure, read back from its acceleration (two particles,
+    no viscosity or gravity: |a| = 2 m (p/rho^2) 45/(pi h^6)(h - d)^2), is
+    B((rho/rho0)^7 - 1) over compressed, near-rest and rarefied densities,
+    clamped to 0 or scaled by the attractive ratio below rest."""
+    h = _f32(0.02)
+    r = _f32(0.4 * h)
+    rho0, B = 1000.0, 3000.0
+    m = rho0 * (2 * r) ** 3
+    seen = set()
+    for attractive in (False, True):
+        for q in (0.2, 0.3, 0.38, 0.45, 0.5, 0.6, 0.8, 0.95):
+            pos, _, _, _ = _pair_case(q, (2.0, 1.0, -2.0), h=0.02)
+            vel = np.zeros((2, 3))
+            rho, acc, _, _ = _oracle_step(pos, vel, kernel_radius=h, particle_radius=r, rest_density=rho0,
+                                          pressure_stiffness=B, viscosity=0.0, gravity=(0, 0, 0),
+                                          use_attractive_pressure=int(attractive),
+                                          attractive_pressure_ratio=0.25)
+            d = np.linalg.norm(pos[0] - pos[1])
+            u = (pos[0] - pos[1]) / d
+            a_along = float(np.dot(acc[0], u))
+            p_oracle = a_along * rho[0] ** 2 / (2.0 * m * 45.0 / (math.pi * h ** 6) * (h - d) ** 2)
+            x = rho[0] / rho0
+            p = B * (x ** 7 - 1.0)
+            if p < 0:
+                p = 0.25 * p if attractive else 0.0
+            assert p_oracle == pytest.approx(p, rel=1e-9, abs=1e-9 * B), (q, attractive)
+            seen.add((x > 1.0, attractive))
+    assert seen == {(True, False), (False, False), (True, True), (False, True)}
+
+
+def test_interior_lattice_density_is_the_closed_lattice_sum(oracle_lib):
+    """A particle inside a perfect lattice of spacing 2r = h/2: its density is
+    m 315/(64 pi h^9) sum over lattice offsets o with |o| < h of
+    (h^2 - |o|^2)^3 = rho0 (315 / 512 pi) (1 + 6 (3/4)^3 + 12 (1/2)^3 + 8 (1/4)^3)
+    = 1.00978 rho0 (the 6 offsets at exactly h add 0).  Checked for two
+    rest densities, so the mass rule m = rho0 (2r)^3 is pinned too."""
+    h = _f32(0.02)
+    r = _f32(h / 4)
+    s = 2.0 * r
+    ijk = np.stack(np.meshgrid(*[np.arange(-3, 4)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    pos = 0.5 + s * ijk.astype(np.float64)
+    centre = int(np.flatnonzero((ijk == 0).all(1))[0])
+    closed = 315.0 / (512.0 * math.pi) * (1 + 6 * 0.75 ** 3 + 12 * 0.5 ** 3 + 8 * 0.25 ** 3)
+    assert closed == pytest.approx(1.00978, abs=1e-5)
+    for rho0 in (1000.0, 750.0):
+        rho, acc, _, _ = _oracle_step(pos, np.zeros_like(pos), kernel_radius=h, particle_radius=r,
+                                      rest_density=rho0, gravity=(0, 0, 0), pressure_stiffness=100.0)
+        assert rho[centre] == pytest.approx(_f32(rho0) * closed, rel=1e-9)
+        # The lattice is symmetric about the centre: no net force there.
+        assert np.abs(acc[centre]).max() < 1e-9 * np.abs(acc).max()
 
 
 def test_momentum_conservation_without_gravity(oracle_lib):
