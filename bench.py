@@ -15,8 +15,9 @@ threads, and one thread).  Extras: `batched` (256-scene ensemble, one launch
 per pass, vs the same scenes launched one engine at a time), `adjoint`
 (rollout reverse pass vs forward wall-clock, 16 x 32,768 particles, 200
 steps, checkpoint every 20), `dambreak_65k` (the 65,536-particle dam break,
-1000 steps), `sphere_4m` (the 4.19M-particle sphere drop over a resting
-layer, r scaled to 0.0015) and `reference_session` (frames per second of the
+1000 steps), `sphere_4m` (the 4.19M-particle sphere drop: a seeded uniform
+sample of a ball with minimum spacing 1.8r over a resting layer, r scaled to
+0.0014) and `reference_session` (frames per second of the
 reference application's default session: DoubleDambreak at resolution 24,
 GUI defaults, 334 steps per 1/30 s frame; `cpu_baseline` times one of its
 frames on the oracle).

@@ -144,6 +144,16 @@ int  sph_set_use_graph(sph_handle* h, int on);
  * particle_radius = (2/res)/4.  Pass xyz_out = NULL to query the count.     */
 int  sph_scene_generate(int scene, int resolution, float* xyz_out, uint32_t capacity,
                         uint32_t* n_out);
+/* Seeded uniform sample of a ball: `count` points inside the ball of
+ * `radius` around centre[3], no two closer than min_spacing (random
+ * sequential addition: uniform darts, each kept when no kept point lies
+ * within min_spacing).  The random counterpart of the SphereDrop lattice
+ * (Source/SceneManager.cpp:67-91) for the sphere-drop workload.  Writes
+ * 3 count floats; *n_out = points placed.  SPH_E_CAPACITY when 64 count
+ * darts leave fewer than count points (the ball is too full at that
+ * spacing); the points placed are written.                                  */
+int  sph_scene_sample_ball(uint64_t seed, const float* centre, float radius, float min_spacing,
+                           uint32_t count, float* xyz_out, uint32_t* n_out);
 
 /* ---- batched ensemble (no reference counterpart: many scenes per launch) ---- */
 typedef struct sph_scene_desc {

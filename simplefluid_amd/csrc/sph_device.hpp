@@ -144,6 +144,13 @@ __device__ __forceinline__ RowGeom row_geom(const SceneConst& sc, float x, float
     g.qz[0] = cz > 0 ? lz * lz : 4.0f;
     g.qz[1] = 0.0f;
     g.qz[2] = cz < sc.G[2] - 1 ? uz * uz : 4.0f;
+    // A particle with a non-finite coordinate has no neighbours (every d^2
+    // with it is NaN, which fails d^2 < h^2): all its windows are empty, so
+    // a pile of such particles in one clamped cell costs nothing here.
+    if (!(__builtin_isfinite(x) && __builtin_isfinite(y) && __builtin_isfinite(z))) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i) g.qy[i] = g.qz[i] = 4.0f;
+    }
     g.row0 = sc.cell_offset + ((uint32_t)cz * (uint32_t)sc.G[1] + (uint32_t)cy) * (uint32_t)sc.G[0];
     return g;
 }

@@ -357,6 +357,9 @@ int check_device_error(sph_handle* h) {
         if (flag & 1u) what += " sort look-back wait exceeded its bound;";
         if (flag & 2u) what += " sort scatter index out of range;";
         if (flag & 4u) what += " sort gather index out of range;";
+        if (flag & 8u)
+            what += " a cell held more than 1024 particles (collapsed or non-finite positions), so its order "
+                    "was the arrival order, not the stable one (SPH_SORT_RADIX has no such limit);";
         return fail(h, SPH_E_DEVICE, "device error flag set:" + what);
     }
     return SPH_OK;

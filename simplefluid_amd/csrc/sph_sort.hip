@@ -275,7 +275,12 @@ __global__ void cell_scatter_kernel(uint32_t n, uint32_t C, const uint32_t* __re
         atomicOr(err, 2u);
 }
 
+// Members a cell may hold for the stable in-cell rank (cell_rank_kernel
+// scans its cell once per member).  Rest-density fluid holds a few per cell.
+constexpr uint32_t kMaxRankCell = 1024;
+
 __global__ void cell_rank_kernel(uint32_t n, uint32_t C, const uint32_t* __restrict__ key0,
+                                 const uint32_t* __restrict__ rank0,
                                  const uint32_t* __restrict__ cstart, const uint32_t* __restrict__ tmp,
                                  const float4* __restrict__ pos_in, const float4* __restrict__ vel_in,
                                  const uint32_t* __restrict__ id_in, float4* __restrict__ pos_out,
@@ -296,8 +301,18 @@ __global__ void cell_rank_kernel(uint32_t n, uint32_t C, const uint32_t* __restr
         return;
     }
     const uint32_t lo = cstart[key], hi = cstart[key + 1];
-    uint32_t r = 0;
-    for (uint32_t t = lo; t < hi; ++t) r += tmp[t] < k ? 1u : 0u;
+    uint32_t r;
+    if (hi - lo <= kMaxRankCell) {
+        r = 0;
+        for (uint32_t t = lo; t < hi; ++t) r += tmp[t] < k ? 1u : 0u;
+    } else {
+        // A cell this full only arises from collapsed or non-finite
+        // positions (clamped into one cell).  Ranking it costs its size per
+        // member, so it keeps the arrival order (a valid permutation, not
+        // the stable one) and raises the error flag instead.
+        r = rank0[k];
+        atomicOr(err, 8u);
+    }
     const uint32_t dst = lo + r;
     if (dst >= hi || hi > n) {
         atomicOr(err, 4u);
@@ -563,7 +578,7 @@ hipError_t launch_sort_gather(hipStream_t st, const SortPlan& sp) {
     const uint32_t tsum_words = scan_tiles(sp.ncells);
     if (sp.mode == kSortCellCount)
         hipLaunchKernelGGL(cell_rank_kernel, dim3(blocks_for(sp.n, sp.reorder_block)), dim3(sp.reorder_block), 0,
-                           st, sp.n, sp.ncells, sp.key0, sp.cstart, sp.tmp, sp.pos_in, sp.vel_in, sp.id_in,
+                           st, sp.n, sp.ncells, sp.key0, sp.rank0, sp.cstart, sp.tmp, sp.pos_in, sp.vel_in, sp.id_in,
                            sp.pos_out, sp.vel_out, sp.id_out, sp.skey, sp.perm, tsum,
                            tsum_words, sp.err, sp.soa_out, sp.halt);
     else

@@ -35,13 +35,16 @@ __device__ __forceinline__ void density_trip(const float4& pi, float h2, const f
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
         // q = d2 - h2 is negative exactly when d2 < h2 (an exact tie rounds
-        // to +0), so its sign bit is the mask bit and -q = h2 - d2 (rounding
-        // is symmetric) is the kernel's t; slots past the row end get q = 1.
+        // to +0), and -q = h2 - d2 (rounding is symmetric) is the kernel's t;
+        // slots past the row end get q = 1.  The mask bit is the comparison
+        // q < 0 itself, not q's sign bit: a NaN distance (a candidate with a
+        // non-finite coordinate) may carry a set sign bit, and it is no
+        // neighbour, as d2 < h2 says.  t = max(NaN, 0) = 0 adds nothing.
         float q = dist2(pi.x - xs[u], pi.y - ys[u], pi.z - zs[u]) - h2;
         if (!FULL && u != 0 && u >= rem) q = 1.0f;
         const float t = fmaxf(-q, 0.0f);
         sum = fmaf(t * t, t, sum);
-        m = (m << 1) | (__float_as_uint(q) >> 31);
+        m = (m << 1) | (q < 0.0f ? 1u : 0u);
     }
 }
 

@@ -47,7 +47,7 @@ EXPORTS = (
     "sph_set_params", "sph_get_params", "sph_set_particles", "sph_make_ready",
     "sph_advance_frame", "sph_advance_steps", "sph_run_frame", "sph_num_particles",
     "sph_get_particles", "sph_get_velocity", "sph_get_density", "sph_export_frame", "sph_snapshot_begin", "sph_snapshot_wait", "sph_synchronize",
-    "sph_last_error", "sph_set_use_graph", "sph_scene_generate", "sph_batch_create",
+    "sph_last_error", "sph_set_use_graph", "sph_scene_generate", "sph_scene_sample_ball", "sph_batch_create",
     "sph_num_scenes", "sph_scene_offsets", "sph_set_scene_params", "sph_rollout_forward",
     "sph_rollout_backward", "sph_profile_enable", "sph_profile_read", "sph_autotune",
     "sph_get_tuning", "sph_set_tuning", "sph_debug_constants", "sph_debug_get_state",
@@ -127,6 +127,7 @@ def load_library(path: str = LIB_PATH):
         "sph_last_error": (C.c_char_p, [H]),
         "sph_set_use_graph": (i32, [H, i32]),
         "sph_scene_generate": (i32, [i32, i32, f32p, u32, P(u32)]),
+        "sph_scene_sample_ball": (i32, [C.c_uint64, f32p, C.c_float, C.c_float, u32, f32p, P(u32)]),
         "sph_batch_create": (i32, [P(sph_scene_desc), u32, P(H)]),
         "sph_num_scenes": (u32, [H]),
         "sph_scene_offsets": (i32, [H, vp]),
@@ -228,6 +229,20 @@ def scene_generate(scene: int, resolution: int = 24) -> np.ndarray:
     rc = L.sph_scene_generate(int(scene), int(resolution), _p(out), n.value, C.byref(n))
     if rc != SPH_OK:
         raise SPHError(rc, "scene_generate")
+    return out
+
+
+def sample_ball(seed: int, centre, radius: float, min_spacing: float, count: int) -> np.ndarray:
+    """`count` seeded uniform points in a ball, no two closer than
+    min_spacing (random sequential addition), as (count, 3) float32."""
+    L = load_library()
+    c = np.ascontiguousarray(centre, np.float32).reshape(3)
+    out = np.empty((int(count), 3), np.float32)
+    n = C.c_uint32(0)
+    rc = L.sph_scene_sample_ball(C.c_uint64(int(seed)), _p(c), float(radius), float(min_spacing), int(count),
+                                 _p(out), C.byref(n))
+    if rc != SPH_OK:
+        raise SPHError(rc, f"sample_ball: placed {n.value} of {count} points (the ball is too full at this spacing)")
     return out
 
 

@@ -312,6 +312,85 @@ def test_double_dambreak_subset_parity(E):
     step_and_compare(E, e, cfg["params"], "double_dambreak slab")
 
 
+@pytest.mark.parametrize("name,steps", [("double_dambreak_1m", 30), ("sphere_drop_4m", 20)])
+def test_full_size_step_parity(E, name, steps):
+    """The whole 1,048,576-particle double dam break and the whole 4.4M
+    sphere drop (4,194,304 sampled points plus the resting layer): after
+    some engine steps, one step from the identical state on the GPU and on
+    the oracle, every particle: cell keys, permutation, cell table and
+    neighbour sets bit-exact, density, acceleration and state within the
+    module's tolerances."""
+    from simplefluid_amd import configs
+    cfg = getattr(configs, name)()
+    p = cfg["params"]
+    say(f"[{name}] {len(cfg['pos'])} particles, {steps} engine steps first")
+    e = make_engine(E, p, cfg["pos"])
+    e.advance_steps(steps)
+    step_and_compare(E, e, p, f"{name} full")
+    e.close()
+
+
+def test_ensemble_256_step_parity(E):
+    """All 256 scenes of the ensemble (5.7M particles, one launch per pass):
+    after 20 batched steps, one more batched step against the oracle run
+    scene by scene from the same state.  Each scene's slice of the batch's
+    keys (less the scene's cell offset), permutation, cell table and
+    neighbour lists (less the scene's first index) equals the oracle's bit
+    for bit; density, acceleration and positions within the tolerances."""
+    import oracle
+    from simplefluid_amd import configs
+    ens = configs.ensemble()
+    params = [s["params"] for s in ens["scenes"]]
+    counts = [s["pos"].shape[0] for s in ens["scenes"]]
+    b = E.BatchEngine(params, counts)
+    b.set_particles(np.concatenate([s["pos"] for s in ens["scenes"]]))
+    b.make_ready()
+    b.advance_steps(20)
+    xs, vs, ids = b.debug_get_state()
+    off = b.scene_offsets()
+    b.debug_keep_acc(True)
+    b.advance_steps(1)
+    g = b.debug_step_outputs(acc=True)
+    gs, ge = b.debug_cells()
+    goff, glst = b.debug_neighbors()
+    xg, vg, ig = b.debug_get_state()
+    say(f"[ensemble] {len(ids)} particles in {len(counts)} scenes, {len(glst)} neighbour entries")
+    coff = 0
+    worst = 0.0
+    for i, s in enumerate(ens["scenes"]):
+        c = b.debug_constants(i)
+        ncell = int(np.prod(c["G"]))
+        a, z = int(off[i]), int(off[i + 1])
+        sl = slice(a, z)
+        o = oracle.Oracle(to_oracle(s["params"], c["xsub"]), threads=16)
+        o.record_neighbors(True)
+        o.set_state(xs[sl].astype(np.float64), vs[sl].astype(np.float64), ids[sl] - a)
+        o.step()
+        d = o.step_debug()
+        assert np.array_equal(g["key"][sl] - coff, d["key"]), f"scene {i} keys"
+        assert np.array_equal(g["perm"][sl] - a, d["perm"]), f"scene {i} perm"
+        os_, oe = o.cells()
+        cs_, ce_ = gs[coff:coff + ncell], ge[coff:coff + ncell]
+        occ = oe > 0
+        assert np.array_equal(occ, ce_ > 0), f"scene {i} occupied cells"
+        assert np.array_equal(cs_[occ] - a, os_[occ]) and np.array_equal(ce_[occ] - a, oe[occ]), f"scene {i} cells"
+        ooff, olst = o.neighbors()
+        assert np.array_equal(goff[a:z + 1] - goff[a], ooff), f"scene {i} neighbour counts"
+        assert np.array_equal(glst[goff[a]:goff[z]] - a, olst), f"scene {i} neighbour sets"
+        assert np.all(np.abs(g["rho"][sl] - d["rho"]) <= RHO_RTOL * np.abs(d["rho"])), f"scene {i} density"
+        scale = np.maximum(np.abs(d["acc"]).max(1, keepdims=True), 10.0)
+        err = (np.abs(g["acc"][sl] - d["acc"]) / scale).max()
+        assert err <= ACC_TOL, f"scene {i} acc {err}"
+        xo, vo, io = o.state()
+        assert np.array_equal(ig[sl] - a, io)
+        assert np.abs(xg[sl] - xo).max() <= POS_ATOL, f"scene {i} positions"
+        worst = max(worst, err)
+        coff += ncell
+    assert coff == len(gs)
+    say(f"[ensemble] all scenes equal the oracle; worst scaled acc err {worst:.2e}")
+    b.close()
+
+
 PARAM_VARIANTS = {
     "attractive": dict(pressure_stiffness=5000.0, viscosity=0.01, use_attractive_pressure=1,
                        attractive_pressure_ratio=0.3),
@@ -338,7 +417,7 @@ def test_parameter_variants_parity(E, variant):
 
 
 def test_sphere_drop_subset_parity(E):
-    """The 4.4M sphere drop's physics (r 0.0015, h 0.006, dt 5e-5) on the
+    """The 4.4M sphere drop's physics (r 0.0014, h 0.0056, dt 5e-5) on the
     particles of its lower corner: the sphere's underside above the resting
     layer on the floor; checked after 20 steps."""
     from simplefluid_amd import configs
@@ -878,6 +957,134 @@ def test_nan_and_out_of_box_particles_are_contained(E):
     ok[5] = False
     assert np.isfinite(x[ok]).all()
     assert (x[6] <= 1.0).all() and (x[6] >= 0.0).all()
+
+
+def _nan32(negative):
+    return np.array([0xFFC00000 if negative else 0x7FC00000], np.uint32).view(np.float32)[0]
+
+
+def test_nan_positions_of_both_signs_inside_the_block(E):
+    """Particles inside the 65k dam-break block get NaN coordinates of both
+    signs, on one axis or all three; a NaN y clamps into the floor's cell
+    row, beside the floor layer's particles.  One step from the same state:
+    keys, permutation, cell table and neighbour sets equal the oracle's (its
+    predicate is d^2 < h^2, which no NaN distance passes), density, force
+    and state agree where finite and are NaN at the same places; particles
+    farther than h from where the NaN particles were get bit for bit what a
+    NaN-free step gives them; and every finite particle stays finite over
+    20 more steps."""
+    import oracle
+    from simplefluid_amd import configs
+    cfg = configs.dambreak_65k(jitter=1e-6, seed=12)
+    p = cfg["params"]
+    pos = cfg["pos"].copy()
+    h = float(p.kernel_radius)
+    plant = [((0.155, 0.305, 0.155), (1,), True), ((0.105, 0.205, 0.205), (1,), False),
+             ((0.205, 0.405, 0.105), (0,), True), ((0.055, 0.105, 0.255), (0, 1, 2), True),
+             ((0.255, 0.505, 0.055), (2,), False), ((0.155, 0.005, 0.155), (1,), True),
+             ((0.305, 0.005, 0.045), (1,), False)]
+    bad = []
+    pos_nan = pos.copy()
+    for c, axes, neg in plant:
+        i = int(np.argmin(((pos - np.asarray(c, np.float32)) ** 2).sum(1)))
+        bad.append(i)
+        for ax in axes:
+            pos_nan[i, ax] = _nan32(neg)
+    bad = np.array(bad)
+    assert np.signbit(pos_nan[bad[0], 1]) and not np.signbit(pos_nan[bad[1], 1])
+    say(f"[nan] NaN particles {bad.tolist()}")
+
+    e = make_engine(E, p, pos_nan)
+    xs, vs, ids = e.debug_get_state()
+    o = oracle.Oracle(to_oracle(p, e.debug_constants()["xsub"]), threads=16)
+    o.record_neighbors(True)
+    o.set_state(xs.astype(np.float64), vs.astype(np.float64), ids)
+    e.debug_keep_acc(True)
+    e.advance_steps(1)
+    g = e.debug_step_outputs(acc=True)
+    o.step()
+    d = o.step_debug()
+    assert np.array_equal(g["key"], d["key"]) and np.array_equal(g["perm"], d["perm"])
+    gs, ge = e.debug_cells()
+    os_, oe = o.cells()
+    assert np.array_equal(gs, os_) and np.array_equal(ge, oe)
+    goff, glst = e.debug_neighbors()
+    ooff, olst = o.neighbors()
+    assert np.array_equal(goff, ooff) and np.array_equal(glst, olst), "neighbour sets differ"
+    sid = d["sorted_id"]
+    nan_sorted = np.isin(sid, bad)
+    # The NaN particles have no neighbours, not even themselves, and are no
+    # finite particle's neighbour.
+    assert (np.diff(goff)[nan_sorted] == 0).all()
+    assert not np.isin(glst, np.flatnonzero(nan_sorted)).any()
+    assert np.all(np.abs(g["rho"] - d["rho"]) <= RHO_RTOL * np.abs(d["rho"]))
+    fin = np.isfinite(d["acc"]).all(1)
+    assert np.array_equal(fin, np.isfinite(g["acc"]).all(1))
+    scale = np.maximum(np.abs(d["acc"][fin]).max(1, keepdims=True), 10.0)
+    assert (np.abs(g["acc"][fin] - d["acc"][fin]) / scale).max() <= ACC_TOL
+    xg, vg, _ = e.debug_get_state()
+    xo, vo, _ = o.state()
+    np.testing.assert_allclose(xg, xo, rtol=0, atol=POS_ATOL, equal_nan=True)
+    e.debug_keep_acc(False)
+
+    # Against the same step without the NaNs, in original order.
+    x_nan, v_nan, rho_nan = e.get_particles(), e.get_velocity(), e.get_density()
+    ref = make_engine(E, p, pos)
+    ref.advance_steps(1)
+    x_ref, v_ref, rho_ref = ref.get_particles(), ref.get_velocity(), ref.get_density()
+    dist = np.sqrt(((pos[:, None, :].astype(np.float64) - pos[bad][None, :, :]) ** 2).sum(2)).min(1)
+    far = dist > 1.01 * h
+    assert far.sum() > 60000
+    assert np.array_equal(x_nan[far], x_ref[far]) and np.array_equal(v_nan[far], v_ref[far])
+    assert np.array_equal(rho_nan[far], rho_ref[far])
+    ok = np.ones(len(pos), bool)
+    ok[bad] = False
+    e.advance_steps(20)
+    x = e.get_particles()
+    assert np.isfinite(x[ok]).all() and np.isfinite(e.get_velocity()[ok]).all()
+    say(f"[nan] ok: {int(far.sum())} far particles bit-identical, {len(glst)} neighbour entries")
+    e.close()
+    ref.close()
+
+
+def test_overfull_cell_is_flagged_and_radix_mode_orders_it(E):
+    """2,000 particles with NaN positions clamp into one cell, past the
+    1,024 members the cell-count sort ranks by scanning the cell: the step
+    stays in bounds, keeps the arrival order there and reports SPH_E_DEVICE
+    saying so; the radix sort has no such limit and equals the oracle."""
+    import oracle
+    from simplefluid_amd import configs
+    p = configs.dambreak_65k()["params"]
+    blk = configs.lattice_block((16, 16, 16), (0.4, 0.4, 0.4))
+    pile = np.full((2000, 3), np.nan, np.float32)
+    pos = np.concatenate([blk, pile])
+    e = make_engine(E, p, pos)
+    e.advance_steps(1)
+    with pytest.raises(E.SPHError) as ei:
+        e.synchronize()
+    assert ei.value.code == E.SPH_E_DEVICE and "more than 1024" in str(ei.value), str(ei.value)
+    e.close()
+    e = E.Engine(p)
+    e.set_particles(pos)
+    e.set_sort_mode(E.SORT_RADIX)
+    e.make_ready()
+    e.advance_steps(2)
+    xs, vs, ids = e.debug_get_state()
+    o = oracle.Oracle(to_oracle(p, e.debug_constants()["xsub"]), threads=16)
+    o.record_neighbors(True)
+    o.set_state(xs.astype(np.float64), vs.astype(np.float64), ids)
+    e.advance_steps(1)
+    g = e.debug_step_outputs()
+    o.step()
+    d = o.step_debug()
+    assert np.array_equal(g["perm"], d["perm"])
+    goff, glst = e.debug_neighbors()
+    ooff, olst = o.neighbors()
+    assert np.array_equal(goff, ooff) and np.array_equal(glst, olst)
+    x = e.get_particles()
+    assert np.isfinite(x[:len(blk)]).all() and np.isnan(x[len(blk):]).all()
+    e.synchronize()
+    e.close()
 
 
 def test_batched_equals_per_scene(E):

@@ -143,3 +143,42 @@ def test_golden_frame_fixture_loads_and_matches_fp32_oracle(lib, oracle_lib):
     xo, vo = o.state_by_id()
     assert np.abs(xo - x).max() < 2e-6
     assert np.abs(vo - v).max() < 1e-4
+
+
+def test_ball_sample_spacing_uniformity_and_seed(lib):
+    """sph_scene_sample_ball: every point inside the ball, no pair closer than
+    the minimum spacing (all pairs checked with a k-d tree, in float64 on the
+    float32 output, with float32 rounding as the only slack), the same points
+    for the same seed and others for another, and uniform: the fraction
+    within half the radius is 1/8 and the octants hold equal shares, within
+    a few standard deviations of a multinomial."""
+    from scipy.spatial import cKDTree
+    from simplefluid_amd import engine
+    c, R, s, n = (0.3, -0.2, 0.1), 0.1, 0.0034, 60000
+    x = engine.sample_ball(7, c, R, s, n)
+    assert x.shape == (n, 3) and x.dtype == np.float32
+    d = x.astype(np.float64) - np.asarray(c, np.float32)
+    rad = np.sqrt((d * d).sum(1))
+    assert rad.max() < R * (1 + 1e-6)
+    pairs = cKDTree(x.astype(np.float64)).query_pairs(s * (1 - 1e-6))
+    assert len(pairs) == 0
+    # The packing is dense: most darts that still fit were placed.
+    phi = n * (np.pi / 6) * s ** 3 / (4 / 3 * np.pi * R ** 3)
+    assert phi > 0.25
+    assert np.array_equal(x, engine.sample_ball(7, c, R, s, n))
+    assert not np.array_equal(x, engine.sample_ball(8, c, R, s, n))
+    inner = (rad < R / 2).mean()
+    # Points are excluded from a shell s/2 thick at the surface less than
+    # inside; allow that bias plus 4 sigma.
+    assert abs(inner - 0.125) < 4 * np.sqrt(0.125 * 0.875 / n) + 0.01
+    octant = (d[:, 0] > 0).astype(int) * 4 + (d[:, 1] > 0) * 2 + (d[:, 2] > 0)
+    share = np.bincount(octant, minlength=8) / n
+    assert np.abs(share - 0.125).max() < 4 * np.sqrt(0.125 * 0.875 / n)
+
+
+def test_ball_sample_too_full_is_refused(lib):
+    """More points than the ball can hold at the spacing: SPH_E_CAPACITY."""
+    from simplefluid_amd import engine
+    with pytest.raises(engine.SPHError) as ei:
+        engine.sample_ball(1, (0, 0, 0), 0.05, 0.01, 200000)
+    assert ei.value.code == engine.SPH_E_CAPACITY

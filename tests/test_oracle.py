@@ -88,6 +88,37 @@ def test_neighbor_sets_equal_bruteforce(oracle_lib, jitter):
     assert all(k in set(lst[off[k]:off[k + 1]]) for k in range(0, n, 97))
 
 
+def test_nan_positions_have_no_neighbours(oracle_lib):
+    """NaN coordinates of both signs inside a lattice (one axis or all three):
+    the cell-list sets still equal the all-pairs sets, the NaN particles have
+    no neighbours (not even themselves) and are nobody's neighbour."""
+    import oracle
+    pos = lattice((10, 12, 9), jitter=0.0005, seed=8).astype(np.float32)
+    nan = {True: np.uint32(0xFFC00000).view(np.float32), False: np.uint32(0x7FC00000).view(np.float32)}
+    bad = [17, 240, 333, 512, 777]
+    for i, (axes, neg) in zip(bad, [((1,), True), ((1,), False), ((0,), True), ((0, 1, 2), True), ((2,), False)]):
+        for ax in axes:
+            pos[i, ax] = nan[neg]
+    assert np.signbit(pos[17, 1]) and np.isnan(pos[17, 1])
+    p = oracle.make_params(x_subdivision=4)
+    o = oracle.Oracle(p)
+    o.record_neighbors(True)
+    o.set_state(pos.astype(np.float64))
+    o.step()
+    off, lst = o.neighbors()
+    d = o.step_debug()
+    sp = pos[d["perm"]]
+    boff, blst = oracle.bruteforce_neighbors(sp, p.kernel_radius)
+    assert np.array_equal(off, boff)
+    for k in range(len(pos)):
+        assert np.array_equal(np.sort(lst[off[k]:off[k + 1]]), blst[boff[k]:boff[k + 1]]), k
+    nan_sorted = np.flatnonzero(np.isin(d["perm"], bad))
+    assert (np.diff(off)[nan_sorted] == 0).all()
+    assert not np.isin(lst, nan_sorted).any()
+    ok = ~np.isin(d["sorted_id"], bad)
+    assert np.isfinite(d["rho"]).all() and np.isfinite(d["acc"][ok]).all()
+
+
 @pytest.mark.parametrize("xsub", [1, 4])
 def test_clustered_cloud_neighbors_equal_bruteforce(oracle_lib, xsub):
     """Random clouds: uniform background, dense Gaussian blobs (cells with tens
