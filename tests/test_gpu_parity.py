@@ -970,9 +970,9 @@ def test_nan_positions_of_both_signs_inside_the_block(E):
     keys, permutation, cell table and neighbour sets equal the oracle's (its
     predicate is d^2 < h^2, which no NaN distance passes), density, force
     and state agree where finite and are NaN at the same places; particles
-    farther than h from where the NaN particles were get bit for bit what a
-    NaN-free step gives them; and every finite particle stays finite over
-    20 more steps."""
+    farther than h from where the NaN particles were get bit for bit the
+    density a NaN-free step gives them (and the state, farther than 2h); and
+    every finite particle stays finite over 20 more steps."""
     import oracle
     from simplefluid_amd import configs
     cfg = configs.dambreak_65k(jitter=1e-6, seed=12)
@@ -1033,16 +1033,19 @@ def test_nan_positions_of_both_signs_inside_the_block(E):
     ref.advance_steps(1)
     x_ref, v_ref, rho_ref = ref.get_particles(), ref.get_velocity(), ref.get_density()
     dist = np.sqrt(((pos[:, None, :].astype(np.float64) - pos[bad][None, :, :]) ** 2).sum(2)).min(1)
+    # Density sees the NaN particles' absence within h of where they were;
+    # force, through the neighbours' densities, within 2h.
     far = dist > 1.01 * h
-    assert far.sum() > 60000
-    assert np.array_equal(x_nan[far], x_ref[far]) and np.array_equal(v_nan[far], v_ref[far])
+    far2 = dist > 2.02 * h
+    assert far2.sum() > 60000
     assert np.array_equal(rho_nan[far], rho_ref[far])
+    assert np.array_equal(x_nan[far2], x_ref[far2]) and np.array_equal(v_nan[far2], v_ref[far2])
     ok = np.ones(len(pos), bool)
     ok[bad] = False
     e.advance_steps(20)
     x = e.get_particles()
     assert np.isfinite(x[ok]).all() and np.isfinite(e.get_velocity()[ok]).all()
-    say(f"[nan] ok: {int(far.sum())} far particles bit-identical, {len(glst)} neighbour entries")
+    say(f"[nan] ok: {int(far.sum())} / {int(far2.sum())} far particles bit-identical, {len(glst)} neighbour entries")
     e.close()
     ref.close()
 
