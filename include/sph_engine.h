@@ -155,6 +155,23 @@ int  sph_scene_generate(int scene, int resolution, float* xyz_out, uint32_t capa
 int  sph_scene_sample_ball(uint64_t seed, const float* centre, float radius, float min_spacing,
                            uint32_t count, float* xyz_out, uint32_t* n_out);
 
+/* ---- one step that allocates nothing ---- */
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc): one step of n particles whose state the caller holds in device
+ * memory, enqueued on `stream` (a hipStream_t; NULL = the default stream),
+ * with every intermediate in a fixed scratch array of the library (about
+ * 17 MB per GPU, held until the process ends).  xyz_in, vel_in (may be NULL:
+ * zero velocity), xyz_out, vel_out: device arrays of 3n floats in the
+ * caller's particle order.  The step and its results equal the first
+ * sph_advance_steps step of a handle given the same particles and params
+ * (default launch shape).  Limits: n <= 65536, fixed dt (cfl_number 0), a
+ * grid of at most 2^20 cells; past them SPH_E_INVALID, with the reason in
+ * sph_last_error(NULL).  Returns without waiting for the GPU.  All calls on
+ * one GPU share its scratch: calls that overlap on one GPU race.            */
+int  sph_step_noalloc(const sph_params* p, const float* xyz_in, const float* vel_in, float* xyz_out,
+                      float* vel_out, uint32_t n, void* stream);
+
 /* ---- batched ensemble (no reference counterpart: many scenes per launch) ---- */
 typedef struct sph_scene_desc {
     sph_params params;               /* per-scene physics, box and dt             */

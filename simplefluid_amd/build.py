@@ -26,6 +26,7 @@ SOURCES = [
     "sph_sort.hip",
     "sph_step.hip",
     "sph_adjoint.hip",
+    "sph_noalloc.hip",
     "sph_engine.cpp",
     "sph_rollout.cpp",
     "sph_autotune.cpp",

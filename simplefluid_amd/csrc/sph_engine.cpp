@@ -26,7 +26,10 @@ namespace sph {
 
 // Message of the last failure on this thread that had no handle to carry it
 // (sph_create, sph_batch_create); sph_last_error(NULL) returns it.
-thread_local std::string t_last_error = "no error";
+// A fixed buffer, so that recording a failure allocates nothing.
+thread_local char t_last_error[512] = "no error";
+
+void set_thread_error(const char* msg) { std::snprintf(t_last_error, sizeof(t_last_error), "%s", msg); }
 thread_local MemBudget* t_budget = nullptr;
 
 int fail(sph_handle* h, int code, const std::string& msg) {
@@ -38,7 +41,7 @@ int fail(sph_handle* h, int code, const std::string& msg) {
     } else if (h) {
         h->last_error = msg;
     } else {
-        t_last_error = msg;
+        set_thread_error(msg.c_str());
     }
     return code;
 }
@@ -57,16 +60,17 @@ static bool finite3(const float* v) { return std::isfinite(v[0]) && std::isfinit
 // whose byte range and offsets are 32-bit: 4 (C + 1) must stay below 2^32.
 constexpr uint64_t kMaxCells = (1ull << 30) - 2;
 
-int derive_scene(const sph_params& p, SceneConst& sc, std::string& err, int xsub) {
-    if (!(p.particle_radius > 0.f) || !std::isfinite(p.particle_radius)) { err = "particle_radius must be > 0"; return SPH_E_INVALID; }
-    if (!(p.kernel_radius > 0.f) || !std::isfinite(p.kernel_radius)) { err = "kernel_radius must be > 0"; return SPH_E_INVALID; }
-    if (!(p.rest_density > 0.f) || !std::isfinite(p.rest_density)) { err = "rest_density must be > 0"; return SPH_E_INVALID; }
-    if (!(p.pressure_stiffness >= 0.f) || !std::isfinite(p.pressure_stiffness)) { err = "pressure_stiffness must be >= 0"; return SPH_E_INVALID; }
-    if (!(p.viscosity >= 0.f) || !std::isfinite(p.viscosity)) { err = "viscosity must be >= 0"; return SPH_E_INVALID; }
-    if (!(p.boundary_restitution >= 0.f) || !std::isfinite(p.boundary_restitution)) { err = "boundary_restitution must be >= 0"; return SPH_E_INVALID; }
-    if (!(p.dt > 0.f) || !std::isfinite(p.dt)) { err = "dt must be > 0"; return SPH_E_INVALID; }
-    if (!finite3(p.gravity) || !finite3(p.box_min) || !finite3(p.box_max)) { err = "gravity and box must be finite"; return SPH_E_INVALID; }
-    if (!std::isfinite(p.attractive_pressure_ratio)) { err = "attractive_pressure_ratio must be finite"; return SPH_E_INVALID; }
+// err receives a static message on failure.
+int derive_scene_core(const sph_params& p, SceneConst& sc, const char** err, int xsub) {
+    if (!(p.particle_radius > 0.f) || !std::isfinite(p.particle_radius)) { *err = "particle_radius must be > 0"; return SPH_E_INVALID; }
+    if (!(p.kernel_radius > 0.f) || !std::isfinite(p.kernel_radius)) { *err = "kernel_radius must be > 0"; return SPH_E_INVALID; }
+    if (!(p.rest_density > 0.f) || !std::isfinite(p.rest_density)) { *err = "rest_density must be > 0"; return SPH_E_INVALID; }
+    if (!(p.pressure_stiffness >= 0.f) || !std::isfinite(p.pressure_stiffness)) { *err = "pressure_stiffness must be >= 0"; return SPH_E_INVALID; }
+    if (!(p.viscosity >= 0.f) || !std::isfinite(p.viscosity)) { *err = "viscosity must be >= 0"; return SPH_E_INVALID; }
+    if (!(p.boundary_restitution >= 0.f) || !std::isfinite(p.boundary_restitution)) { *err = "boundary_restitution must be >= 0"; return SPH_E_INVALID; }
+    if (!(p.dt > 0.f) || !std::isfinite(p.dt)) { *err = "dt must be > 0"; return SPH_E_INVALID; }
+    if (!finite3(p.gravity) || !finite3(p.box_min) || !finite3(p.box_max)) { *err = "gravity and box must be finite"; return SPH_E_INVALID; }
+    if (!std::isfinite(p.attractive_pressure_ratio)) { *err = "attractive_pressure_ratio must be finite"; return SPH_E_INVALID; }
     std::memset(&sc, 0, sizeof(sc));
     const float r = p.particle_radius;
     const float sp = 2.0f * r;
@@ -82,7 +86,7 @@ int derive_scene(const sph_params& p, SceneConst& sc, std::string& err, int xsub
     sc.cV = (float)((double)p.viscosity * (double)mass * spiky);
     sc.inv_rho0 = (float)(1.0 / (double)p.rest_density);
     sc.inv_cs = (float)(1.0 / (hd * (1.0 + 1.0 / 1024.0)));
-    if (xsub != 1 && xsub != 2 && xsub != 4) { err = "x subdivision must be 1, 2 or 4"; return SPH_E_INVALID; }
+    if (xsub != 1 && xsub != 2 && xsub != 4) { *err = "x subdivision must be 1, 2 or 4"; return SPH_E_INVALID; }
     sc.inv_csx = sc.inv_cs * (float)xsub;  // exact: power-of-two scale
     sc.xsubf = (float)xsub;
     {
@@ -96,7 +100,7 @@ int derive_scene(const sph_params& p, SceneConst& sc, std::string& err, int xsub
     sc.e = p.boundary_restitution;
     sc.dt = p.dt;
     sc.dt_max = p.dt;
-    if (!(p.cfl_number >= 0.f) || !std::isfinite(p.cfl_number)) { err = "cfl_number must be >= 0"; return SPH_E_INVALID; }
+    if (!(p.cfl_number >= 0.f) || !std::isfinite(p.cfl_number)) { *err = "cfl_number must be >= 0"; return SPH_E_INVALID; }
     sc.cfl = p.cfl_number;
     sc.c0 = (float)std::sqrt(7.0 * (double)p.pressure_stiffness / (double)p.rest_density);
     sc.mu = p.viscosity;
@@ -109,11 +113,11 @@ int derive_scene(const sph_params& p, SceneConst& sc, std::string& err, int xsub
         sc.bmin[k] = p.box_min[k];
         sc.wmax[k] = p.box_max[k];
         const float ext = p.box_max[k] - p.box_min[k];
-        if (!(ext > 2.0f * r)) { err = "box must be wider than 2*particle_radius on every axis"; return SPH_E_INVALID; }
+        if (!(ext > 2.0f * r)) { *err = "box must be wider than 2*particle_radius on every axis"; return SPH_E_INVALID; }
         int G = (int)std::ceil(ext * (k == 0 ? sc.inv_csx : sc.inv_cs));
         if (G < 1) G = 1;
         if (G > (k == 0 ? 1024 * xsub : 1024)) {
-            err = "grid exceeds 1024 cells on an axis (box / kernel_radius too large)";
+            *err = "grid exceeds 1024 cells on an axis (box / kernel_radius too large)";
             return SPH_E_INVALID;
         }
         sc.G[k] = G;
@@ -121,8 +125,20 @@ int derive_scene(const sph_params& p, SceneConst& sc, std::string& err, int xsub
         sc.hi[k] = p.box_max[k] - r;
         cells *= (uint64_t)G;
     }
-    if (cells > kMaxCells) { err = "too many grid cells (limit 2^30 - 2)"; return SPH_E_INVALID; }
+    if (cells > kMaxCells) { *err = "too many grid cells (limit 2^30 - 2)"; return SPH_E_INVALID; }
     return SPH_OK;
+}
+
+int derive_scene(const sph_params& p, SceneConst& sc, std::string& err, int xsub) {
+    const char* msg = nullptr;
+    const int rc = derive_scene_core(p, sc, &msg, xsub);
+    if (rc != SPH_OK) err = msg;
+    return rc;
+}
+
+int derive_scene_noalloc(const sph_params& p, SceneConst& sc, int xsub) {
+    const char* msg = nullptr;
+    return derive_scene_core(p, sc, &msg, xsub);
 }
 
 void invalidate_graph(sph_handle* h) {
@@ -889,7 +905,7 @@ int sph_synchronize(sph_handle* h) {
     return sync(h);
 }
 
-const char* sph_last_error(const sph_handle* h) { return h ? h->last_error.c_str() : t_last_error.c_str(); }
+const char* sph_last_error(const sph_handle* h) { return h ? h->last_error.c_str() : t_last_error; }
 
 int sph_set_use_graph(sph_handle* h, int on) {
     BudgetScope budget_scope(h ? &h->mem : nullptr);

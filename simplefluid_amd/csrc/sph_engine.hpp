@@ -201,6 +201,10 @@ namespace sph {
 int fail(sph_handle* h, int code, const std::string& msg);
 int hip_fail(sph_handle* h, hipError_t e, const char* what);
 int derive_scene(const sph_params& p, SceneConst& sc, std::string& err, int xsub);
+// The same checks and constants without building a message (allocates nothing).
+int derive_scene_noalloc(const sph_params& p, SceneConst& sc, int xsub);
+// Records the failure message of a call that has no handle (sph_last_error(NULL)).
+void set_thread_error(const char* msg);
 int alloc_step_bufs(sph_handle* h, StepBufs& w);
 void invalidate_graph(sph_handle* h);
 uint2* tiles_for(sph_handle* h, int block, uint32_t& ntiles);

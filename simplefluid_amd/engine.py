@@ -54,7 +54,7 @@ EXPORTS = (
     "sph_debug_set_state", "sph_debug_keep_acc", "sph_debug_step_outputs",
     "sph_debug_num_cells", "sph_debug_cells", "sph_debug_neighbors", "sph_set_sort_mode",
     "sph_get_sort_mode", "sph_set_cell_subdivision", "sph_set_device_memory_limit",
-    "sph_device_memory_used",
+    "sph_device_memory_used", "sph_step_noalloc",
 )
 SORT_CELL_COUNT = 0
 SORT_RADIX = 1
@@ -152,6 +152,7 @@ def load_library(path: str = LIB_PATH):
         "sph_set_cell_subdivision": (i32, [H, i32]),
         "sph_set_device_memory_limit": (i32, [H, C.c_uint64]),
         "sph_device_memory_used": (i32, [H, P(C.c_uint64)]),
+        "sph_step_noalloc": (i32, [P(sph_params), vp, vp, vp, vp, u32, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
