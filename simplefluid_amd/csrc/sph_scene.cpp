@@ -113,7 +113,7 @@ extern "C" int sph_scene_generate(int scene, int resolution, float* xyz_out, uin
 // a grid of min_spacing cells (a dart checks the 27 around its own) as
 // per-cell linked lists.
 //
-// Darts come in batches of 2^22, each visited in cell order.  The grid is cut
+// Darts come in batches (4 count, within [2^16, 2^22]), each visited in cell order.  The grid is cut
 // into slabs two cells thick in z; a dart only reads cells within one cell
 // of its own, so darts of slabs two apart never see each other's points, and
 // a batch runs in two phases (even slabs, then odd), each slab's darts in
@@ -175,7 +175,7 @@ extern "C" int sph_scene_sample_ball(uint64_t seed, const float* centre, float r
     std::vector<int32_t> head(plane * G, -1);
     std::vector<Slab> slabs(nslab);
     Xoshiro256 rng(seed);
-    const uint32_t batch = 1u << 22;
+    const uint32_t batch = (uint32_t)std::min<uint64_t>(1u << 22, std::max<uint64_t>(1u << 16, 4ull * count));
     const uint64_t max_darts = 64ull * count + batch;
     auto cell_of = [&](double v, int k) {
         const int q = (int)std::floor((v - (c[k] - R)) / s);
@@ -234,9 +234,11 @@ extern "C" int sph_scene_sample_ball(uint64_t seed, const float* centre, float r
             for (uint32_t i = i0; i < i1;) {
                 const double x = 2.0 * r.uniform() - 1.0, y = 2.0 * r.uniform() - 1.0, z = 2.0 * r.uniform() - 1.0;
                 if (x * x + y * y + z * z >= 1.0) continue;
-                d[3 * (size_t)i] = c[0] + R * x;
-                d[3 * (size_t)i + 1] = c[1] + R * y;
-                d[3 * (size_t)i + 2] = c[2] + R * z;
+                // Rounded to float first: the spacing is checked on the
+                // coordinates the caller receives.
+                d[3 * (size_t)i] = (double)(float)(c[0] + R * x);
+                d[3 * (size_t)i + 1] = (double)(float)(c[1] + R * y);
+                d[3 * (size_t)i + 2] = (double)(float)(c[2] + R * z);
                 order[i].first = ((uint64_t)cell_of(d[3 * (size_t)i + 2], 2) * G +
                                   (uint64_t)cell_of(d[3 * (size_t)i + 1], 1)) * G +
                                  (uint64_t)cell_of(d[3 * (size_t)i], 0);

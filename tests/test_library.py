@@ -146,9 +146,9 @@ def test_golden_frame_fixture_loads_and_matches_fp32_oracle(lib, oracle_lib):
 
 
 def test_ball_sample_spacing_uniformity_and_seed(lib):
-    """sph_scene_sample_ball: every point inside the ball, no pair closer than
-    the minimum spacing (all pairs checked with a k-d tree, in float64 on the
-    float32 output, with float32 rounding as the only slack), the same points
+    """sph_scene_sample_ball: every point inside the ball, no pair of the
+    float32 output closer than the minimum spacing (all pairs checked with a
+    k-d tree in float64), the same points
     for the same seed and others for another, and uniform: the fraction
     within half the radius is 1/8 and the octants hold equal shares, within
     a few standard deviations of a multinomial."""
@@ -160,7 +160,10 @@ def test_ball_sample_spacing_uniformity_and_seed(lib):
     d = x.astype(np.float64) - np.asarray(c, np.float32)
     rad = np.sqrt((d * d).sum(1))
     assert rad.max() < R * (1 + 1e-6)
-    pairs = cKDTree(x.astype(np.float64)).query_pairs(s * (1 - 1e-6))
+    # The sampler checks the spacing on the float32 coordinates it returns,
+    # against the float32 spacing.
+    s32 = float(np.float32(s))
+    pairs = cKDTree(x.astype(np.float64)).query_pairs(s32 * (1 - 1e-12))
     assert len(pairs) == 0
     # The packing is dense: most darts that still fit were placed.
     phi = n * (np.pi / 6) * s ** 3 / (4 / 3 * np.pi * R ** 3)
