@@ -17,10 +17,11 @@ per pass, vs the same scenes launched one engine at a time), `adjoint`
 steps, checkpoint every 20), `dambreak_65k` (the 65,536-particle dam break,
 1000 steps), `sphere_4m` (the 4.19M-particle sphere drop: a seeded uniform
 sample of a ball with minimum spacing 1.8r over a resting layer, r scaled to
-0.0014) and `reference_session` (frames per second of the
+0.0014), `reference_session` (frames per second of the
 reference application's default session: DoubleDambreak at resolution 24,
 GUI defaults, 334 steps per 1/30 s frame; `cpu_baseline` times one of its
-frames on the oracle).
+frames on the oracle) and `slab_1_of_8` (one GPU's share of the 1M scene
+sharded 8 ways by x slabs: the fullest slab with its 2h ghost bands).
 """
 from __future__ import annotations
 
@@ -347,11 +348,39 @@ def sphere_4m_leg(args):
     return out
 
 
+def slab_leg(args):
+    """One GPU's share of the 1M double dam break sharded 8 ways by x slabs
+    (cuts at particle-count quantiles, a 2h ghost band each inner side): the
+    fullest slab, owned particles and ghosts stepped as one scene.  The ghost
+    band's exchange between GPUs is not part of it."""
+    from simplefluid_amd import configs, engine
+    cfg = configs.slab_share(configs.double_dambreak_1m(), 8)
+    e = engine.Engine(cfg["params"])
+    e.set_particles(cfg["pos"])
+    e.make_ready()
+    tune_leg(e, args)
+    e.advance_steps(args.warmup)
+    t = time_steps(e, args.steps)
+    e.profile_enable(True)
+    e.advance_steps(100)
+    ms, nst = e.profile_read()
+    e.profile_enable(False)
+    out = {"shards": 8, "slab": cfg["shard"], "x_range": [round(v, 5) for v in cfg["x_range"]],
+           "owned": cfg["owned"], "ghosts": cfg["ghosts"], "particles": int(cfg["pos"].shape[0]),
+           "steps": args.steps, "ms_per_step": round(t / args.steps * 1e3, 5),
+           "owned_particle_steps_per_s": round(cfg["owned"] * args.steps / t, 1),
+           "pass_ms": {k: round(v / max(nst, 1), 5) for k, v in ms.items()}, "tuning": e.get_tuning()}
+    e.close()
+    log("slab_1_of_8", out)
+    return out
+
+
 def small_and_large(args):
-    """The 65k dam break over its full 1000 steps, the 4.4M sphere drop, and
-    the reference application's default session."""
+    """The 65k dam break over its full 1000 steps, the 4.4M sphere drop, the
+    reference application's default session, and one GPU's slab of the 1M
+    scene sharded 8 ways."""
     return {"dambreak_65k": dambreak_65k_leg(args), "sphere_4m": sphere_4m_leg(args),
-            "reference_session": reference_session_leg(args)}
+            "reference_session": reference_session_leg(args), "slab_1_of_8": slab_leg(args)}
 
 
 REF_FRAMES = 30
@@ -493,7 +522,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
-    ap.add_argument("--only", choices=["plain", "batched", "adjoint", "dambreak65k", "sphere4m", "reference"],
+    ap.add_argument("--only", choices=["plain", "batched", "adjoint", "dambreak65k", "sphere4m", "reference",
+                                       "slab"],
                     default=None,
                     help="run one leg alone (profiling)")
     ap.add_argument("--no-autotune", action="store_true")
@@ -528,6 +558,9 @@ def main():
         return
     if args.only == "reference":
         print(json.dumps({"reference_session": reference_session_leg(args)}), flush=True)
+        return
+    if args.only == "slab":
+        print(json.dumps({"slab_1_of_8": slab_leg(args)}), flush=True)
         return
     value, step_ms, roof, extra, cfg = plain(args)
     out = {"metric": METRIC, "value": round(value, 1), "unit": "particle-steps/s", "n_gpus": 1,

@@ -89,6 +89,44 @@ def sphere_drop_4m(seed=2, n_sphere=4_194_304, r=0.0014):
     return dict(name="sphere_drop_4m", params=p, pos=pos, steps=300, n_sphere=n_sphere)
 
 
+def slab_share(cfg, nshards=8, shard=None):
+    """One GPU's share of a scene sharded by slabs along x: cuts at
+    particle-count quantiles, so each slab owns N / nshards particles, plus a
+    ghost band 2h wide on each inner side (ghosts recompute their own
+    density, so the band must hold every neighbour of an owned particle's
+    neighbours).  shard=None picks the slab with the most owned + ghost
+    particles.  Returned as a standalone scene whose box spans the slab and
+    its bands (2r wider, so no particle starts clamped), x extent only."""
+    pos = cfg["pos"]
+    p = cfg["params"]
+    h = float(p.kernel_radius)
+    r = float(p.particle_radius)
+    bmin = [float(v) for v in p.box_min]
+    bmax = [float(v) for v in p.box_max]
+    n = pos.shape[0]
+    xs = np.sort(pos[:, 0])
+    cuts = [bmin[0]] + [float(xs[(n * s) // nshards]) for s in range(1, nshards)] + [math.inf]
+    shares = []
+    for s in range(nshards):
+        x0, x1 = cuts[s], cuts[s + 1]
+        own = (pos[:, 0] >= x0) & (pos[:, 0] < x1)
+        band = (pos[:, 0] >= x0 - 2 * h) & (pos[:, 0] < x1 + 2 * h) & ~own
+        shares.append((int(own.sum()) + int(band.sum()), s, x0, x1, own, band))
+    if shard is None:
+        _, shard, x0, x1, own, band = max(shares, key=lambda t: t[0])
+    else:
+        _, shard, x0, x1, own, band = shares[shard]
+    # Owned particles first, then the ghosts.
+    sel = np.concatenate([pos[own], pos[band]])
+    lo = max(bmin[0], x0 - 2 * h - 2 * r)
+    hi = min(bmax[0], x1 + 2 * h + 2 * r)
+    q = type(p).from_buffer_copy(p)
+    q.box_min[0], q.box_max[0] = lo, hi
+    return dict(name=f"{cfg['name']}_slab{shard}of{nshards}", params=q, pos=np.ascontiguousarray(sel),
+                owned=int(own.sum()), ghosts=int(band.sum()), shard=shard, x_range=(x0, min(x1, bmax[0])),
+                steps=cfg["steps"])
+
+
 def _block_dims(n, aspect):
     nx = max(1, int(round((n / aspect) ** (1.0 / 3.0))))
     ny = max(1, int(round(n / (nx * nx))))

@@ -108,6 +108,43 @@ def test_synthetic_configs_shapes():
         assert (s["pos"] < 1.0).all()
 
 
+def test_slab_share_partitions_and_holds_every_second_neighbour():
+    """The slabs of an 8-way x sharding own disjoint eighths of the scene, and
+    each share holds every particle within 2h of one it owns (the neighbours of
+    its neighbours, whose densities the share recomputes)."""
+    from scipy.spatial import cKDTree
+    from simplefluid_amd import configs
+    c = configs.dambreak_65k(jitter=0.002, seed=5)
+    pos = c["pos"]
+    h = float(c["params"].kernel_radius)
+    tree = cKDTree(pos.astype(np.float64))
+    seen = np.zeros(len(pos), bool)
+    for s in range(8):
+        sh = configs.slab_share(c, 8, s)
+        assert sh["owned"] == len(pos) // 8
+        own, sel = sh["pos"][:sh["owned"]], sh["pos"]
+        idx = {tuple(p) for p in sel.tolist()}
+        assert len(idx) == len(sel)
+        x0, x1 = sh["x_range"]
+        assert ((own[:, 0] >= x0) & (own[:, 0] < x1)).all()
+        oi = np.flatnonzero((pos[:, 0] >= x0) & (pos[:, 0] < (x1 if s < 7 else np.inf)))
+        assert not seen[oi].any()
+        seen[oi] = True
+        need = set()
+        for lst in tree.query_ball_point(own.astype(np.float64), 2 * h):
+            need.update(lst)
+        assert {tuple(p) for p in pos[sorted(need)].tolist()} <= idx
+        # Nothing farther than 2h from the slab is carried.
+        g = sh["pos"][sh["owned"]:, 0]
+        assert ((g >= x0 - 2 * h) & (g < x1 + 2 * h)).all()
+        lo, hi = sh["params"].box_min[0], sh["params"].box_max[0]
+        assert (sel[:, 0] > lo).all() and (sel[:, 0] < hi).all()
+    assert seen.all()
+    worst = configs.slab_share(c, 8)
+    assert worst["owned"] + worst["ghosts"] == max(
+        configs.slab_share(c, 8, s)["pos"].shape[0] for s in range(8))
+
+
 def test_null_handle_is_rejected(lib):
     """Entry points given no handle return SPH_E_INVALID without touching a GPU."""
     from simplefluid_amd import engine
