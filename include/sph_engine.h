@@ -167,8 +167,11 @@ int  sph_scene_sample_ball(uint64_t seed, const float* centre, float radius, flo
  * sph_advance_steps step of a handle given the same particles and params
  * (default launch shape).  Limits: n <= 65536, fixed dt (cfl_number 0), a
  * grid of at most 2^20 cells; past them SPH_E_INVALID, with the reason in
- * sph_last_error(NULL).  Returns without waiting for the GPU.  All calls on
- * one GPU share its scratch: calls that overlap on one GPU race.            */
+ * sph_last_error(NULL).  Returns without waiting for the GPU, so the
+ * device-side checks a handle reports at its next synchronising call (a
+ * cell of more than 1024 particles, keys out of range) are not reported
+ * here.  All calls on one GPU share its scratch: calls that overlap on one
+ * GPU race.                                                                 */
 int  sph_step_noalloc(const sph_params* p, const float* xyz_in, const float* vel_in, float* xyz_out,
                       float* vel_out, uint32_t n, void* stream);
 
