@@ -531,8 +531,8 @@ def main():
                     help="adjoint leg: steps between checkpoints (default: the workload's 20; "
                          "200 tapes every step and recomputes nothing)")
     ap.add_argument("--tuning", type=str, default="",
-                    help="density,force,reorder blocks, sort items [,persistent 0/1], e.g. 256,128,512,4,0 "
-                         "(skips autotune)")
+                    help="density,force,reorder blocks, sort items [,persistent 0/1 [,staged force 0/1]], "
+                         "e.g. 512,256,256,8,0,1 (skips autotune)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run a 256 MiB device copy (known bytes) for counter calibration, then exit")
     args = ap.parse_args()

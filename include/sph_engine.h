@@ -230,8 +230,11 @@ int  sph_profile_read(sph_handle* h, double* ms_per_pass, uint32_t* steps);
  * keyed by (device arch, particle-count bucket, scene count).               */
 int  sph_autotune(sph_handle* h);
 /* Current choice (5 ints): block sizes of density, force and reorder, the
- * sort items per thread, and 1 when density and force run as persistent
- * blocks taking tiles from per-XCD queues (0: one block per tile).          */
+ * sort items per thread, and the launch form: bit 0 set when density and
+ * force run as persistent blocks taking tiles from per-XCD queues (clear:
+ * one block per tile), bit 1 set when the force pass stages each row's
+ * candidates in LDS (clear: it gathers them from global memory).  Every
+ * choice gives the same results bit for bit.                                */
 int  sph_get_tuning(const sph_handle* h, int* cfg5);
 int  sph_set_tuning(sph_handle* h, const int* cfg5);
 /* Sort used to order particles by cell (no reference counterpart; both give

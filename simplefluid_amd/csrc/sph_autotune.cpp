@@ -1,6 +1,7 @@
 // Autotuner: picks the block size of the density, force and reorder passes,
-// the keys per thread of the sort's digit passes and whether density and
-// force run as persistent blocks over tile queues, by timing each candidate
+// the keys per thread of the sort's digit passes, whether density and force
+// run as persistent blocks over tile queues and whether the force pass
+// stages its rows in LDS, by timing each candidate
 // with HIP events on the handle's own state.  The state is saved first and
 // restored before every trial, so all candidates run the same steps.
 // Results are cached per process, keyed by device arch, particle-count
@@ -113,19 +114,22 @@ extern "C" int sph_autotune(sph_handle* h) {
         *ax.field = best_v;
         if (rc != SPH_OK) break;
     }
-    // Launch form of density and force, judged on their summed time.
+    // Launch form of density and force (persistent or one block per tile;
+    // force rows staged in LDS or gathered), judged on their summed time.
     if (rc == SPH_OK) {
         double best_ms = 1e30;
         int best_v = 0;
-        for (int c = 0; c < 2; ++c) {
-            best.persistent = c;
+        for (int c = 0; c < 4; ++c) {
+            best.persistent = c & 1;
+            best.force_staged = c >> 1;
             h->tune = best;
             rc = trial(ms);
             if (rc != SPH_OK) break;
             const double t = ms[SPH_PASS_DENSITY] + ms[SPH_PASS_FORCE];
             if (t < best_ms) { best_ms = t; best_v = c; }
         }
-        best.persistent = best_v;
+        best.persistent = best_v & 1;
+        best.force_staged = best_v >> 1;
     }
     h->tune = best;
     h->prof = prof_was;

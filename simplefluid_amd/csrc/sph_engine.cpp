@@ -170,17 +170,21 @@ uint2* tiles_for(sph_handle* h, int block, uint32_t& ntiles) {
     return d.p;
 }
 
-// Persistent launches keep about 2048 lanes per CU resident and loop over
-// tile queues; they are used only when there are more tiles than that.
-static uint32_t resident_blocks(const sph_handle* h, int block) {
-    return (uint32_t)std::max(8, h->cu_count) * (uint32_t)std::max(1, 2048 / block);
+// Persistent launches keep as many blocks resident as the kernel's
+// occupancy allows (2048 lanes per CU, or what the staged force pass's LDS
+// and registers leave) and loop over tile queues; they are used only when
+// there are more tiles than that.
+static uint32_t resident_blocks(const sph_handle* h, int block, int blocks_per_cu) {
+    const int per_cu = blocks_per_cu > 0 ? blocks_per_cu : std::max(1, 2048 / block);
+    return (uint32_t)std::max(8, h->cu_count) * (uint32_t)per_cu;
 }
 
-uint32_t grid_for(sph_handle* h, int block, uint32_t ntiles, uint32_t** queue) {
+uint32_t grid_for(sph_handle* h, int block, uint32_t ntiles, uint32_t** queue, int blocks_per_cu) {
     if (queue) *queue = nullptr;
-    if (!h->tune.persistent || !queue || ntiles <= resident_blocks(h, block)) return ntiles;
+    const uint32_t resident = resident_blocks(h, block, blocks_per_cu);
+    if (!h->tune.persistent || !queue || ntiles <= resident) return ntiles;
     *queue = h->queue.p;
-    return resident_blocks(h, block);
+    return resident;
 }
 
 StepArgs step_args(sph_handle* h, StepBufs& w, int block) {
@@ -351,8 +355,9 @@ int enqueue_step(sph_handle* h, StepBufs& w, bool record_prof, uint32_t* frame) 
         StepArgs a = step_args(h, w, h->tune.force_block);
         a.vmax2 = h->cfl ? h->vmax2.p : nullptr;
         a.halt = frame;
-        const uint32_t grid = grid_for(h, h->tune.force_block, a.ntiles, &a.queue);
-        e = launch_force(st, h->tune.force_block, grid, a);
+        const uint32_t grid = grid_for(h, h->tune.force_block, a.ntiles, &a.queue,
+                                       force_blocks_per_cu(h->tune.force_block, h->tune.force_staged));
+        e = launch_force(st, h->tune.force_block, grid, a, h->tune.force_staged);
         if (e != hipSuccess) return hip_fail(h, e, "force launch");
     }
     if (record_prof) {
@@ -631,6 +636,7 @@ static int capture_graph(sph_handle* h, bool gated = false) {
         uint32_t nt = 0;
         if (!tiles_for(h, b, nt)) return fail(h, SPH_E_OOM, "tile table");
     }
+    (void)force_blocks_per_cu(h->tune.force_block, h->tune.force_staged);  // runtime query, outside capture
     hipError_t e = hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return hip_fail(h, e, "begin capture");
     const int rc = enqueue_step(h, h->work, false, gated ? h->frame.p : nullptr);
@@ -1028,7 +1034,7 @@ int sph_get_tuning(const sph_handle* h, int* cfg) {
     cfg[1] = h->tune.force_block;
     cfg[2] = h->tune.reorder_block;
     cfg[3] = h->tune.sort_items;
-    cfg[4] = h->tune.persistent;
+    cfg[4] = h->tune.persistent | (h->tune.force_staged << 1);
     return SPH_OK;
 }
 
@@ -1041,12 +1047,14 @@ int sph_set_tuning(sph_handle* h, const int* cfg) {
         return fail(h, SPH_E_INVALID, "block sizes must be 64, 128, 256 or 512");
     if (cfg[3] != 4 && cfg[3] != 8 && cfg[3] != 12 && cfg[3] != 16)
         return fail(h, SPH_E_INVALID, "sort items must be 4, 8, 12 or 16");
-    if (cfg[4] != 0 && cfg[4] != 1) return fail(h, SPH_E_INVALID, "persistent must be 0 or 1");
+    if (cfg[4] < 0 || cfg[4] > 3)
+        return fail(h, SPH_E_INVALID, "launch form must be 0 to 3 (bit 0 persistent, bit 1 staged force)");
     h->tune.density_block = cfg[0];
     h->tune.force_block = cfg[1];
     h->tune.reorder_block = cfg[2];
     h->tune.sort_items = cfg[3];
-    h->tune.persistent = cfg[4];
+    h->tune.persistent = cfg[4] & 1;
+    h->tune.force_staged = (cfg[4] >> 1) & 1;
     invalidate_graph(h);
     return SPH_OK;
 }

@@ -92,6 +92,7 @@ struct Tuning {
     int reorder_block = 256;
     int sort_items = 8;
     int persistent = 0;   // density and force: 0 one block per tile, 1 persistent blocks over tile queues
+    int force_staged = 0; // force: 1 each row's candidates staged in LDS, 0 gathered from global memory
     int sort_mode = kSortCellCount;
 };
 
@@ -210,7 +211,8 @@ void invalidate_graph(sph_handle* h);
 uint2* tiles_for(sph_handle* h, int block, uint32_t& ntiles);
 // Grid of a density or force launch; *queue receives the tile counters when
 // the launch is persistent, else null.
-uint32_t grid_for(sph_handle* h, int block, uint32_t ntiles, uint32_t** queue);
+// blocks_per_cu: the kernel's resident blocks per CU (0: 2048 lanes' worth).
+uint32_t grid_for(sph_handle* h, int block, uint32_t ntiles, uint32_t** queue, int blocks_per_cu = 0);
 StepArgs step_args(sph_handle* h, StepBufs& w, int block);
 // Enqueue one step reading/writing the state buffers with intermediates in w.
 // frame: device frame state (every kernel of the step skips once it is

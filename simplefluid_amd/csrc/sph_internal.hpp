@@ -104,7 +104,11 @@ constexpr int kFrameWords = 4;
 hipError_t launch_frame_reset(hipStream_t st, uint32_t* frame);
 hipError_t launch_frame_gate(hipStream_t st, uint32_t* frame);
 hipError_t launch_density(hipStream_t st, int block, uint32_t grid, const StepArgs& a);
-hipError_t launch_force(hipStream_t st, int block, uint32_t grid, const StepArgs& a);
+// staged: the force pass reads each row's candidates from LDS (same results).
+hipError_t launch_force(hipStream_t st, int block, uint32_t grid, const StepArgs& a, int staged = 0);
+// Resident blocks per CU of the force kernel at this block size and form
+// (the runtime's occupancy figure; 0 if it cannot say).
+int force_blocks_per_cu(int block, int staged);
 hipError_t launch_fill_u32(hipStream_t st, uint32_t* p, uint32_t value, size_t count);
 // Debug: neighbour CSR as the force pass sees it.
 hipError_t launch_neighbors(hipStream_t st, int block, uint32_t grid, const StepArgs& a,

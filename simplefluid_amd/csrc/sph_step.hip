@@ -8,6 +8,8 @@
 // row is the contiguous sorted range of the x cells inside the support
 // sphere's chord at that row; ascending index.  Density sums and force sums
 // run in that order, one particle per lane.
+#include <atomic>
+
 #include "sph_internal.hpp"
 
 namespace sph {
@@ -118,6 +120,9 @@ __device__ __forceinline__ void force_pair(const SceneConst& sc, const float4& p
     f.vz = fmaf(vv, vj.z - vi.z, f.vz);
 }
 
+__device__ __forceinline__ uint32_t force_finish(const StepArgs& a, const SceneConst& sc, uint32_t k,
+                                                 const float4& pi, const float4& vi, const ForceAcc& f);
+
 __device__ __forceinline__ uint32_t force_one(const StepArgs& a, const SceneConst& sc, uint32_t k,
                                               __amdgpu_buffer_rsrc_t rpos, __amdgpu_buffer_rsrc_t rvel,
                                               __amdgpu_buffer_rsrc_t rcs) {
@@ -147,6 +152,12 @@ __device__ __forceinline__ uint32_t force_one(const StepArgs& a, const SceneCons
                 force_pair(sc, pi, vi, load4(rpos, j << 4), load4(rvel, j << 4), f);
             });
     }
+    return force_finish(a, sc, k, pi, vi, f);
+}
+
+// Walls, integrate, clamp and stores of particle k; returns its next key.
+__device__ __forceinline__ uint32_t force_finish(const StepArgs& a, const SceneConst& sc, uint32_t k,
+                                                 const float4& pi, const float4& vi, const ForceAcc& f) {
     const float cvi = sc.cV * vi.w;
     // The walls' pressure term at this particle's own p/rho^2.
     const float cwi = sc.cW * pi.w;
@@ -175,6 +186,113 @@ __device__ __forceinline__ uint32_t force_one(const StepArgs& a, const SceneCons
     a.vel_o[k] = make_float4(v[0], v[1], v[2], 0.0f);
     a.id_o[k] = a.id_s[k];
     return cell_key(sc, p[0], p[1], p[2]);
+}
+
+// Wave-wide minimum / maximum of one value per lane (all 64 lanes active):
+// DPP within each row of 16 lanes, then the four rows' results.
+__device__ __forceinline__ uint32_t row16_min(uint32_t v) {
+    v = min(v, (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0xB1, 0xF, 0xF, false));   // quad_perm 1,0,3,2
+    v = min(v, (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x4E, 0xF, 0xF, false));   // quad_perm 2,3,0,1
+    v = min(v, (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x141, 0xF, 0xF, false));  // row_half_mirror
+    v = min(v, (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x140, 0xF, 0xF, false));  // row_mirror
+    return v;
+}
+__device__ __forceinline__ uint32_t row16_max(uint32_t v) {
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0xB1, 0xF, 0xF, false));
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x4E, 0xF, 0xF, false));
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x141, 0xF, 0xF, false));
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x140, 0xF, 0xF, false));
+    return v;
+}
+__device__ __forceinline__ uint32_t wave_umin(uint32_t v) {
+    v = row16_min(v);
+    const uint32_t a0 = (uint32_t)__builtin_amdgcn_readlane((int)v, 0), a1 = (uint32_t)__builtin_amdgcn_readlane((int)v, 16);
+    const uint32_t a2 = (uint32_t)__builtin_amdgcn_readlane((int)v, 32), a3 = (uint32_t)__builtin_amdgcn_readlane((int)v, 48);
+    return min(min(a0, a1), min(a2, a3));
+}
+__device__ __forceinline__ uint32_t wave_umax(uint32_t v) {
+    v = row16_max(v);
+    const uint32_t a0 = (uint32_t)__builtin_amdgcn_readlane((int)v, 0), a1 = (uint32_t)__builtin_amdgcn_readlane((int)v, 16);
+    const uint32_t a2 = (uint32_t)__builtin_amdgcn_readlane((int)v, 32), a3 = (uint32_t)__builtin_amdgcn_readlane((int)v, 48);
+    return max(max(a0, a1), max(a2, a3));
+}
+
+// LDS order between a wave's own stores and its later loads of other
+// lanes' entries (and back).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Candidates a wave stages per row: its lanes' windows of one row lie in
+// one sorted range of about 64 + 2 chords of particles when the wave sits in
+// one y/z column; a wider range (a wave across two columns) is walked from
+// global memory as in force_one.
+constexpr uint32_t kStage = 128;
+struct StageSlot {
+    float4 pos[kStage];
+    float4 vel[kStage];
+};
+
+// force_one with each row's candidates read from LDS: per row, the wave
+// copies the union of its lanes' windows (contiguous, so the copy is
+// coalesced) into its slot, then every lane walks its own mask over the
+// slot.  Same pairs, same order, same arithmetic: the results are
+// force_one's bit for bit.  Every lane of the wave calls it (the copy and
+// the wave-wide bounds need them all); inactive lanes have empty windows.
+__device__ __forceinline__ uint32_t force_one_staged(const StepArgs& a, const SceneConst& sc, uint32_t k,
+                                                     bool active, __amdgpu_buffer_rsrc_t rpos,
+                                                     __amdgpu_buffer_rsrc_t rvel, __amdgpu_buffer_rsrc_t rcs,
+                                                     StageSlot& slot) {
+    // Loads past the buffer end return zeros.
+    const float4 pi = load4(rpos, k << 4);
+    const float4 vi = load4(rvel, k << 4);
+    const RowGeom g = row_geom(sc, pi.x, pi.y, pi.z);
+    uint32_t lo[kRows], hi[kRows], mk[kRows];
+#pragma unroll
+    for (int row = 0; row < kRows; ++row) {
+        row_window(rcs, sc, g, row % 3 - 1, row / 3 - 1, lo[row], hi[row]);
+        mk[row] = active ? a.mask[(size_t)row * a.n + k] : 0u;
+        if (!active) lo[row] = hi[row] = 0u;
+    }
+    {
+        const uint32_t b = mask_bit(lo[4], hi[4], k);
+        if (active && b < (uint32_t)kMaskBits) mk[4] &= ~(1u << b);
+    }
+    const uint32_t lane = threadIdx.x & 63u;
+    ForceAcc f = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int row = 0; row < kRows; ++row) {
+        const bool has = hi[row] > lo[row];
+        const uint32_t ulo = wave_umin(has ? lo[row] : 0xffffffffu);
+        const uint32_t uhi = wave_umax(has ? hi[row] : 0u);
+        if (ulo >= uhi) continue;
+        if (uhi - ulo <= kStage) {
+            for (uint32_t e = lane; e < uhi - ulo; e += 64u) {
+                slot.pos[e] = load4(rpos, (ulo + e) << 4);
+                slot.vel[e] = load4(rvel, (ulo + e) << 4);
+            }
+            wave_lds_sync();
+            row_neighbors(
+                lo[row], hi[row], mk[row],
+                [&](uint32_t j) {
+                    const float4 pj = slot.pos[j - ulo];
+                    return j != k && dist2(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z) < sc.h2;
+                },
+                [&](uint32_t j) { force_pair(sc, pi, vi, slot.pos[j - ulo], slot.vel[j - ulo], f); });
+            wave_lds_sync();
+        } else {
+            row_neighbors(
+                lo[row], hi[row], mk[row],
+                [&](uint32_t j) {
+                    const float4 pj = load4(rpos, j << 4);
+                    return j != k && dist2(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z) < sc.h2;
+                },
+                [&](uint32_t j) { force_pair(sc, pi, vi, load4(rpos, j << 4), load4(rvel, j << 4), f); });
+        }
+    }
+    return active ? force_finish(a, sc, k, pi, vi, f) : 0u;
 }
 
 template <int BLOCK>
@@ -220,6 +338,35 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8, 8))) v
         if (active) {
             a.key0[k] = key;
             a.rank0[k] = slot;
+        }
+    });
+}
+
+// force_kernel with the rows staged in LDS (4 KB per wave).
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void force_staged_kernel(StepArgs a) {
+    if (a.halt && *a.halt) return;
+    __shared__ StageSlot s_slot[BLOCK / 64];
+    const __amdgpu_buffer_rsrc_t rpos = buffer_of(a.pos_s, a.n);
+    const __amdgpu_buffer_rsrc_t rvel = buffer_of(a.vel_s, a.n);
+    const __amdgpu_buffer_rsrc_t rcs = buffer_of(a.cstart, a.ncells + 1u);
+    uint32_t* queue = a.queue ? a.queue + 8 : nullptr;
+    StageSlot& slot = s_slot[threadIdx.x >> 6];
+    for_each_tile(a.ntiles, queue, [&](uint32_t t) {
+        const uint2 ti = a.tiles[t];
+        const SceneConst& sc = a.scenes[ti.x];
+        const uint32_t k = ti.y + threadIdx.x;
+        const bool active = k < sc.p_end;
+        const uint32_t key = force_one_staged(a, sc, k, active, rpos, rvel, rcs, slot);
+        const uint32_t cslot = count_slot(a.count, a.count + a.ncells, key, active);
+        if (a.vmax2) {
+            float4 vn = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (active) vn = a.vel_o[k];
+            wave_max_v2(a.vmax2, ti.x, vn.x, vn.y, vn.z, active);
+        }
+        if (active) {
+            a.key0[k] = key;
+            a.rank0[k] = cslot;
         }
     });
 }
@@ -369,10 +516,38 @@ hipError_t launch_density(hipStream_t st, int block, uint32_t grid, const StepAr
     return hipGetLastError();
 }
 
-hipError_t launch_force(hipStream_t st, int block, uint32_t grid, const StepArgs& a) {
+hipError_t launch_force(hipStream_t st, int block, uint32_t grid, const StepArgs& a, int staged) {
     if (grid == 0) return hipSuccess;
-    SPH_DISPATCH_BLOCK(force_kernel, block, grid, st, a);
+    if (staged) {
+        SPH_DISPATCH_BLOCK(force_staged_kernel, block, grid, st, a);
+    } else {
+        SPH_DISPATCH_BLOCK(force_kernel, block, grid, st, a);
+    }
     return hipGetLastError();
+}
+
+template <class K>
+static int occupancy(K kernel, int block) {
+    int nb = 0;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, block, 0) == hipSuccess ? nb : 0;
+}
+
+int force_blocks_per_cu(int block, int staged) {
+    // Looked up once per form and block size; 0 = not yet (stores nb + 1).
+    static std::atomic<int> cache[2][4];
+    const int b = block == 64 ? 0 : block == 128 ? 1 : block == 512 ? 3 : 2;
+    std::atomic<int>& c = cache[staged ? 1 : 0][b];
+    const int known = c.load(std::memory_order_relaxed);
+    if (known > 0) return known - 1;
+    int nb = 0;
+    switch (b) {
+        case 0: nb = staged ? occupancy(force_staged_kernel<64>, 64) : occupancy(force_kernel<64>, 64); break;
+        case 1: nb = staged ? occupancy(force_staged_kernel<128>, 128) : occupancy(force_kernel<128>, 128); break;
+        case 3: nb = staged ? occupancy(force_staged_kernel<512>, 512) : occupancy(force_kernel<512>, 512); break;
+        default: nb = staged ? occupancy(force_staged_kernel<256>, 256) : occupancy(force_kernel<256>, 256); break;
+    }
+    c.store(nb + 1, std::memory_order_relaxed);
+    return nb;
 }
 
 hipError_t launch_hash(hipStream_t st, int block, uint32_t grid, const float4* pos, const float4* vel,

@@ -289,15 +289,25 @@ def test_clustered_cloud_parity(E):
 
 
 def test_rows_longer_than_the_mask_parity(E):
+    """Rows of more than 32 candidates (their head re-tested in the force
+    pass), with the force pass gathering from global memory and with its rows
+    staged in LDS (rows up to 128 candidates staged, longer ones gathered);
+    the two forms give the same bits."""
     params, pos = compressed_block_scene()
     for xsub in (1, 4):
-        e = E.Engine(params)
-        e.set_particles(pos)
-        e.set_cell_subdivision(xsub)
-        e.make_ready()
-        d = step_and_compare(E, e, params, f"compressed block xsub {xsub}", vs_fp64=True)
-        step_and_compare(E, e, params, f"compressed block xsub {xsub} step 2", vs_fp64=True)
-        e.close()
+        out = []
+        for staged in (0, 1):
+            e = E.Engine(params)
+            e.set_particles(pos)
+            e.set_cell_subdivision(xsub)
+            e.make_ready()
+            e.set_tuning(256, 256, 256, 8, 0, staged)
+            d = step_and_compare(E, e, params, f"compressed block xsub {xsub} staged {staged}", vs_fp64=True)
+            step_and_compare(E, e, params, f"compressed block xsub {xsub} staged {staged} step 2", vs_fp64=True)
+            e.advance_steps(3)
+            out.append((e.get_particles(), e.get_velocity()))
+            e.close()
+        assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
     assert np.abs(d["acc"]).max() > 1e3
 
 
@@ -312,19 +322,23 @@ def test_double_dambreak_subset_parity(E):
     step_and_compare(E, e, cfg["params"], "double_dambreak slab")
 
 
-@pytest.mark.parametrize("name,steps", [("double_dambreak_1m", 30), ("sphere_drop_4m", 20)])
-def test_full_size_step_parity(E, name, steps):
+@pytest.mark.parametrize("name,steps,staged", [("double_dambreak_1m", 30, 0), ("sphere_drop_4m", 20, 0),
+                                               ("double_dambreak_1m", 30, 1), ("sphere_drop_4m", 20, 1)])
+def test_full_size_step_parity(E, name, steps, staged):
     """The whole 1,048,576-particle double dam break and the whole 4.4M
     sphere drop (4,194,304 sampled points plus the resting layer): after
     some engine steps, one step from the identical state on the GPU and on
     the oracle, every particle: cell keys, permutation, cell table and
     neighbour sets bit-exact, density, acceleration and state within the
-    module's tolerances."""
+    module's tolerances.  staged: the force pass reads each row's candidates
+    from LDS."""
     from simplefluid_amd import configs
     cfg = getattr(configs, name)()
     p = cfg["params"]
-    say(f"[{name}] {len(cfg['pos'])} particles, {steps} engine steps first")
+    say(f"[{name}] {len(cfg['pos'])} particles, {steps} engine steps first, staged force {staged}")
     e = make_engine(E, p, cfg["pos"])
+    e.set_tuning(256, 256, 256, 8, 0, staged)
+    assert e.get_tuning()["force_staged"] == staged
     e.advance_steps(steps)
     step_and_compare(E, e, p, f"{name} full")
     e.close()
@@ -500,7 +514,8 @@ def test_runs_are_deterministic_and_tuning_invariant(E):
     from simplefluid_amd import configs
     cfg = configs.dambreak_65k(jitter=1e-6, seed=4)
     res = []
-    for tune in ((256, 256, 256, 8), (256, 256, 256, 8), (64, 512, 128, 16), (512, 128, 64, 4, 1)):
+    for tune in ((256, 256, 256, 8), (256, 256, 256, 8), (64, 512, 128, 16), (512, 128, 64, 4, 1),
+                 (256, 256, 256, 8, 0, 1), (64, 64, 128, 16, 1, 1), (512, 512, 256, 8, 0, 1)):
         e = make_engine(E, cfg["params"], cfg["pos"])
         e.set_tuning(*tune)
         e.advance_steps(30)
@@ -517,7 +532,8 @@ def test_persistent_launch_matches_one_block_per_tile(E):
     from simplefluid_amd import configs
     cfg = configs.double_dambreak_1m()
     res = []
-    for tune in ((64, 64, 256, 8, 0), (64, 64, 256, 8, 1), (512, 256, 256, 8, 1)):
+    for tune in ((64, 64, 256, 8, 0), (64, 64, 256, 8, 1), (512, 256, 256, 8, 1), (512, 64, 256, 8, 1, 1),
+                 (512, 512, 256, 8, 0, 1)):
         e = make_engine(E, cfg["params"], cfg["pos"])
         e.set_tuning(*tune)
         e.advance_steps(8)
@@ -587,15 +603,17 @@ def test_cfl_frames_on_device_match_the_host_loop(E):
 
 
 def test_tuning_changes_between_captured_frames(E):
-    """Block sizes changed between frames of the adaptive step's device loop
-    (each change re-captures the step graph, with tile tables for block sizes
-    not used before) give the bits of a run with fixed tuning."""
+    """Block sizes and launch forms changed between frames of the adaptive
+    step's device loop (each change re-captures the step graph, with tile
+    tables for block sizes not used before) give the bits of a run with
+    fixed tuning."""
     p = E.default_params(24)
     p.dt = 5e-3
     p.cfl_number = 0.4
     pos = E.scene_generate(E.SCENE_DAMBREAK, 24)
     res = []
-    for plan in (((256, 256, 256, 8),), ((64, 128, 256, 8), (512, 64, 256, 8, 1), (128, 512, 256, 8))):
+    for plan in (((256, 256, 256, 8),), ((64, 128, 256, 8), (512, 64, 256, 8, 1), (128, 512, 256, 8)),
+                 ((64, 128, 256, 8, 0, 1), (512, 64, 256, 8, 1, 1), (128, 512, 256, 8))):
         e = make_engine(E, p, pos)
         frames = []
         for i in range(6):
@@ -603,8 +621,9 @@ def test_tuning_changes_between_captured_frames(E):
             frames.append(e.run_frame())
         res.append((frames, e.get_particles(), e.get_velocity()))
         e.close()
-    assert res[0][0] == res[1][0]
-    assert np.array_equal(res[0][1], res[1][1]) and np.array_equal(res[0][2], res[1][2])
+    for r in res[1:]:
+        assert res[0][0] == r[0]
+        assert np.array_equal(res[0][1], r[1]) and np.array_equal(res[0][2], r[2])
 
 
 def test_cfl_step_parity(E):
@@ -1140,22 +1159,24 @@ def test_batched_with_empty_scenes(E):
 def test_batched_persistent_queues_match(E):
     """A batch of six ~100k-particle scenes (more 64-lane tiles than resident
     blocks, so the per-XCD tile queues are in use) gives the same bits with
-    persistent blocks as with one block per tile."""
+    persistent blocks, and with the force pass staging its rows in LDS, as
+    with one block per tile."""
     from simplefluid_amd import configs
     ens = configs.ensemble(nscenes=6, seed=12, nmin=90000, nmax=110000)
     params = [s["params"] for s in ens["scenes"]]
     counts = [s["pos"].shape[0] for s in ens["scenes"]]
     assert sum(counts) // 64 > 256 * 32
     out = []
-    for persistent in (0, 1):
+    for persistent, staged in ((0, 0), (1, 0), (1, 1)):
         b = E.BatchEngine(params, counts)
         b.set_particles(np.concatenate([s["pos"] for s in ens["scenes"]]))
         b.make_ready()
-        b.set_tuning(64, 64, 256, 8, persistent)
+        b.set_tuning(64, 64, 256, 8, persistent, staged)
         b.advance_steps(15)
         out.append((b.get_particles(), b.get_velocity()))
         b.close()
-    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    for r in out[1:]:
+        assert np.array_equal(out[0][0], r[0]) and np.array_equal(out[0][1], r[1])
 
 
 def test_batched_parity_vs_oracle(E):
