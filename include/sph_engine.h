@@ -233,8 +233,9 @@ int  sph_autotune(sph_handle* h);
  * sort items per thread, and the launch form: bit 0 set when density and
  * force run as persistent blocks taking tiles from per-XCD queues (clear:
  * one block per tile), bit 1 set when the force pass stages each row's
- * candidates in LDS (clear: it gathers them from global memory).  Every
- * choice gives the same results bit for bit.                                */
+ * candidates in LDS (clear: it gathers them from global memory), bit 2 the
+ * same for the density pass.  Every choice gives the same results bit for
+ * bit.                                                                      */
 int  sph_get_tuning(const sph_handle* h, int* cfg5);
 int  sph_set_tuning(sph_handle* h, const int* cfg5);
 /* Sort used to order particles by cell (no reference counterpart; both give

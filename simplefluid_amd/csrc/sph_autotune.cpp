@@ -1,7 +1,7 @@
 // Autotuner: picks the block size of the density, force and reorder passes,
 // the keys per thread of the sort's digit passes, whether density and force
-// run as persistent blocks over tile queues and whether the force pass
-// stages its rows in LDS, by timing each candidate
+// run as persistent blocks over tile queues and whether the force and
+// density passes stage their rows in LDS, by timing each candidate
 // with HIP events on the handle's own state.  The state is saved first and
 // restored before every trial, so all candidates run the same steps.
 // Results are cached per process, keyed by device arch, particle-count
@@ -130,6 +130,19 @@ extern "C" int sph_autotune(sph_handle* h) {
         }
         best.persistent = best_v & 1;
         best.force_staged = best_v >> 1;
+    }
+    // Density rows staged in LDS or read from global memory.
+    if (rc == SPH_OK) {
+        double best_ms = 1e30;
+        int best_v = 0;
+        for (int c = 0; c < 2; ++c) {
+            best.density_staged = c;
+            h->tune = best;
+            rc = trial(ms);
+            if (rc != SPH_OK) break;
+            if (ms[SPH_PASS_DENSITY] < best_ms) { best_ms = ms[SPH_PASS_DENSITY]; best_v = c; }
+        }
+        best.density_staged = best_v;
     }
     h->tune = best;
     h->prof = prof_was;

@@ -342,7 +342,7 @@ int enqueue_step(sph_handle* h, StepBufs& w, bool record_prof, uint32_t* frame) 
         StepArgs a = step_args(h, w, h->tune.density_block);
         a.halt = frame;
         const uint32_t grid = grid_for(h, h->tune.density_block, a.ntiles, &a.queue);
-        e = launch_density(st, h->tune.density_block, grid, a);
+        e = launch_density(st, h->tune.density_block, grid, a, h->tune.density_staged);
         if (e != hipSuccess) return hip_fail(h, e, "density launch");
     }
     if (record_prof) prof_record(h, 4);
@@ -1034,7 +1034,7 @@ int sph_get_tuning(const sph_handle* h, int* cfg) {
     cfg[1] = h->tune.force_block;
     cfg[2] = h->tune.reorder_block;
     cfg[3] = h->tune.sort_items;
-    cfg[4] = h->tune.persistent | (h->tune.force_staged << 1);
+    cfg[4] = h->tune.persistent | (h->tune.force_staged << 1) | (h->tune.density_staged << 2);
     return SPH_OK;
 }
 
@@ -1047,14 +1047,16 @@ int sph_set_tuning(sph_handle* h, const int* cfg) {
         return fail(h, SPH_E_INVALID, "block sizes must be 64, 128, 256 or 512");
     if (cfg[3] != 4 && cfg[3] != 8 && cfg[3] != 12 && cfg[3] != 16)
         return fail(h, SPH_E_INVALID, "sort items must be 4, 8, 12 or 16");
-    if (cfg[4] < 0 || cfg[4] > 3)
-        return fail(h, SPH_E_INVALID, "launch form must be 0 to 3 (bit 0 persistent, bit 1 staged force)");
+    if (cfg[4] < 0 || cfg[4] > 7)
+        return fail(h, SPH_E_INVALID,
+                    "launch form must be 0 to 7 (bit 0 persistent, bit 1 staged force, bit 2 staged density)");
     h->tune.density_block = cfg[0];
     h->tune.force_block = cfg[1];
     h->tune.reorder_block = cfg[2];
     h->tune.sort_items = cfg[3];
     h->tune.persistent = cfg[4] & 1;
     h->tune.force_staged = (cfg[4] >> 1) & 1;
+    h->tune.density_staged = (cfg[4] >> 2) & 1;
     invalidate_graph(h);
     return SPH_OK;
 }

@@ -93,6 +93,7 @@ struct Tuning {
     int sort_items = 8;
     int persistent = 0;   // density and force: 0 one block per tile, 1 persistent blocks over tile queues
     int force_staged = 0; // force: 1 each row's candidates staged in LDS, 0 gathered from global memory
+    int density_staged = 0;  // density: the same choice
     int sort_mode = kSortCellCount;
 };
 

@@ -103,7 +103,8 @@ hipError_t launch_cfl(hipStream_t st, uint32_t nscenes, SceneConst* scenes, uint
 constexpr int kFrameWords = 4;
 hipError_t launch_frame_reset(hipStream_t st, uint32_t* frame);
 hipError_t launch_frame_gate(hipStream_t st, uint32_t* frame);
-hipError_t launch_density(hipStream_t st, int block, uint32_t grid, const StepArgs& a);
+// staged: the density pass reads each row's candidates from LDS (same results).
+hipError_t launch_density(hipStream_t st, int block, uint32_t grid, const StepArgs& a, int staged = 0);
 // staged: the force pass reads each row's candidates from LDS (same results).
 hipError_t launch_force(hipStream_t st, int block, uint32_t grid, const StepArgs& a, int staged = 0);
 // Resident blocks per CU of the force kernel at this block size and form

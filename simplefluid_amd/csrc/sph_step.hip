@@ -25,6 +25,14 @@ __device__ __forceinline__ void tile_span(uint32_t ntiles, uint32_t& t0, uint32_
     t1 = t0 + per < ntiles ? t0 + per : ntiles;
 }
 
+// m = (m << 1) | (q < 0), as a compare into VCC and one add with carry
+// (m + m + carry): two instructions where the compiler's select, shift and
+// or take three.  q < 0 is false for a NaN, as d^2 < h^2 is.
+__device__ __forceinline__ uint32_t push_bit(uint32_t m, float q) {
+    asm("v_cmp_gt_f32 vcc, 0, %1\n\tv_addc_co_u32 %0, vcc, %0, %0, vcc" : "+v"(m) : "v"(q) : "vcc");
+    return m;
+}
+
 // Candidates come from the x, y, z planes: one 16-byte load per plane gives
 // four consecutive candidates (three loads per trip instead of four, which
 // relieves the texture address units this pass otherwise saturates).
@@ -46,7 +54,7 @@ __device__ __forceinline__ void density_trip(const float4& pi, float h2, const f
         if (!FULL && u != 0 && u >= rem) q = 1.0f;
         const float t = fmaxf(-q, 0.0f);
         sum = fmaf(t * t, t, sum);
-        m = (m << 1) | (q < 0.0f ? 1u : 0u);
+        m = push_bit(m, q);
     }
 }
 
@@ -90,6 +98,127 @@ __device__ __forceinline__ void density_one(const StepArgs& a, const SceneConst&
     // p/rho^2 and 1/rho ride in the unused w of the sorted position and
     // velocity, so the force pass gets them with the same loads.  Neighbour
     // reads of pos_s in this pass take x, y, z only.
+    reinterpret_cast<float*>(a.pos_s + k)[3] = (p * ir) * ir;
+    reinterpret_cast<float*>(a.vel_s + k)[3] = ir;
+}
+
+// Wave-wide minimum / maximum of one value per lane (all 64 lanes active):
+// DPP within each row of 16 lanes, then the four rows' results.
+__device__ __forceinline__ uint32_t row16_min(uint32_t v) {
+    v = min(v, (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0xB1, 0xF, 0xF, false));   // quad_perm 1,0,3,2
+    v = min(v, (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x4E, 0xF, 0xF, false));   // quad_perm 2,3,0,1
+    v = min(v, (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x141, 0xF, 0xF, false));  // row_half_mirror
+    v = min(v, (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x140, 0xF, 0xF, false));  // row_mirror
+    return v;
+}
+__device__ __forceinline__ uint32_t row16_max(uint32_t v) {
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0xB1, 0xF, 0xF, false));
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x4E, 0xF, 0xF, false));
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x141, 0xF, 0xF, false));
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x140, 0xF, 0xF, false));
+    return v;
+}
+__device__ __forceinline__ uint32_t wave_umin(uint32_t v) {
+    v = row16_min(v);
+    const uint32_t a0 = (uint32_t)__builtin_amdgcn_readlane((int)v, 0), a1 = (uint32_t)__builtin_amdgcn_readlane((int)v, 16);
+    const uint32_t a2 = (uint32_t)__builtin_amdgcn_readlane((int)v, 32), a3 = (uint32_t)__builtin_amdgcn_readlane((int)v, 48);
+    return min(min(a0, a1), min(a2, a3));
+}
+__device__ __forceinline__ uint32_t wave_umax(uint32_t v) {
+    v = row16_max(v);
+    const uint32_t a0 = (uint32_t)__builtin_amdgcn_readlane((int)v, 0), a1 = (uint32_t)__builtin_amdgcn_readlane((int)v, 16);
+    const uint32_t a2 = (uint32_t)__builtin_amdgcn_readlane((int)v, 32), a3 = (uint32_t)__builtin_amdgcn_readlane((int)v, 48);
+    return max(max(a0, a1), max(a2, a3));
+}
+
+// LDS order between a wave's own stores and its later loads of other
+// lanes' entries (and back).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Candidates a wave stages per row in the density pass, plus the three
+// slots a row's last, partial trip reads past its end (masked out).
+constexpr uint32_t kStageD = 128;
+struct DensitySlot {
+    float x[kStageD + 4];
+    float y[kStageD + 4];
+    float z[kStageD + 4];
+};
+
+// density_one with each row's candidates read from LDS: per row the wave
+// copies the x, y, z planes of the union of its lanes' windows into its slot
+// (coalesced), then every lane runs its own four-candidate trips over the
+// slot.  Same candidates, same order, same arithmetic: density_one's results
+// bit for bit.  Every lane of the wave calls it; inactive lanes have empty
+// windows and store nothing.
+__device__ __forceinline__ void density_one_staged(const StepArgs& a, const SceneConst& sc, uint32_t k, bool active,
+                                                   __amdgpu_buffer_rsrc_t rpos, __amdgpu_buffer_rsrc_t rx,
+                                                   __amdgpu_buffer_rsrc_t ry, __amdgpu_buffer_rsrc_t rz,
+                                                   __amdgpu_buffer_rsrc_t rcs, DensitySlot& slot) {
+    const float4 pi = load4(rpos, k << 4);  // zeros past the buffer end
+    const RowGeom g = row_geom(sc, pi.x, pi.y, pi.z);
+    uint32_t lo[kRows], hi[kRows];
+#pragma unroll
+    for (int row = 0; row < kRows; ++row) {
+        row_window(rcs, sc, g, row % 3 - 1, row / 3 - 1, lo[row], hi[row]);
+        if (!active) lo[row] = hi[row] = 0u;
+    }
+    const uint32_t lane = threadIdx.x & 63u;
+    const float h2 = sc.h2;
+    float sum = 0.0f;
+#pragma unroll
+    for (int row = 0; row < kRows; ++row) {
+        uint32_t m = 0u;
+        const uint32_t cnt = hi[row] - lo[row];
+        const bool has = cnt > 0u;
+        const uint32_t ulo = wave_umin(has ? lo[row] : 0xffffffffu);
+        const uint32_t uhi = wave_umax(has ? hi[row] : 0u);
+        if (ulo < uhi && uhi - ulo <= kStageD) {
+            for (uint32_t e = lane; e < uhi - ulo; e += 64u) {
+                const uint32_t off = (ulo + e) << 2;
+                slot.x[e] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rx, (int)off, 0, 0));
+                slot.y[e] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(ry, (int)off, 0, 0));
+                slot.z[e] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rz, (int)off, 0, 0));
+            }
+            wave_lds_sync();
+            uint32_t o = lo[row] - ulo;
+            const uint32_t full_end = o + (cnt & ~3u);
+            for (; o != full_end; o += 4u)
+                density_trip<true>(pi, h2, make_float4(slot.x[o], slot.x[o + 1], slot.x[o + 2], slot.x[o + 3]),
+                                   make_float4(slot.y[o], slot.y[o + 1], slot.y[o + 2], slot.y[o + 3]),
+                                   make_float4(slot.z[o], slot.z[o + 1], slot.z[o + 2], slot.z[o + 3]), 4, sum, m);
+            const int rem = (int)(cnt & 3u);
+            if (rem > 0)
+                density_trip<false>(pi, h2, make_float4(slot.x[o], slot.x[o + 1], slot.x[o + 2], slot.x[o + 3]),
+                                    make_float4(slot.y[o], slot.y[o + 1], slot.y[o + 2], slot.y[o + 3]),
+                                    make_float4(slot.z[o], slot.z[o + 1], slot.z[o + 2], slot.z[o + 3]), rem, sum,
+                                    m);
+            wave_lds_sync();
+        } else if (has) {
+            uint32_t off = lo[row] << 2;
+            const uint32_t full_end = off + ((cnt & ~3u) << 2);
+            for (; off != full_end; off += 16u)
+                density_trip<true>(pi, h2, load4(rx, off), load4(ry, off), load4(rz, off), 4, sum, m);
+            const int rem = (int)(cnt & 3u);
+            if (rem > 0) density_trip<false>(pi, h2, load4(rx, off), load4(ry, off), load4(rz, off), rem, sum, m);
+        }
+        if (active) a.mask[(size_t)row * a.n + k] = m;
+    }
+    if (!active) return;
+    float wr = 0.0f;
+    const float pk[3] = {pi.x, pi.y, pi.z};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        wr += wall_rho((pk[d] - sc.bmin[d]) * sc.inv_h);
+        wr += wall_rho((sc.wmax[d] - pk[d]) * sc.inv_h);
+    }
+    const float rho = sc.cD * sum + sc.rho0 * wr;
+    const float p = eos(sc, rho);
+    const float ir = 1.0f / rho;
+    a.rho[k] = rho;
     reinterpret_cast<float*>(a.pos_s + k)[3] = (p * ir) * ir;
     reinterpret_cast<float*>(a.vel_s + k)[3] = ir;
 }
@@ -188,43 +317,6 @@ __device__ __forceinline__ uint32_t force_finish(const StepArgs& a, const SceneC
     return cell_key(sc, p[0], p[1], p[2]);
 }
 
-// Wave-wide minimum / maximum of one value per lane (all 64 lanes active):
-// DPP within each row of 16 lanes, then the four rows' results.
-__device__ __forceinline__ uint32_t row16_min(uint32_t v) {
-    v = min(v, (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0xB1, 0xF, 0xF, false));   // quad_perm 1,0,3,2
-    v = min(v, (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x4E, 0xF, 0xF, false));   // quad_perm 2,3,0,1
-    v = min(v, (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x141, 0xF, 0xF, false));  // row_half_mirror
-    v = min(v, (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x140, 0xF, 0xF, false));  // row_mirror
-    return v;
-}
-__device__ __forceinline__ uint32_t row16_max(uint32_t v) {
-    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0xB1, 0xF, 0xF, false));
-    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x4E, 0xF, 0xF, false));
-    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x141, 0xF, 0xF, false));
-    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x140, 0xF, 0xF, false));
-    return v;
-}
-__device__ __forceinline__ uint32_t wave_umin(uint32_t v) {
-    v = row16_min(v);
-    const uint32_t a0 = (uint32_t)__builtin_amdgcn_readlane((int)v, 0), a1 = (uint32_t)__builtin_amdgcn_readlane((int)v, 16);
-    const uint32_t a2 = (uint32_t)__builtin_amdgcn_readlane((int)v, 32), a3 = (uint32_t)__builtin_amdgcn_readlane((int)v, 48);
-    return min(min(a0, a1), min(a2, a3));
-}
-__device__ __forceinline__ uint32_t wave_umax(uint32_t v) {
-    v = row16_max(v);
-    const uint32_t a0 = (uint32_t)__builtin_amdgcn_readlane((int)v, 0), a1 = (uint32_t)__builtin_amdgcn_readlane((int)v, 16);
-    const uint32_t a2 = (uint32_t)__builtin_amdgcn_readlane((int)v, 32), a3 = (uint32_t)__builtin_amdgcn_readlane((int)v, 48);
-    return max(max(a0, a1), max(a2, a3));
-}
-
-// LDS order between a wave's own stores and its later loads of other
-// lanes' entries (and back).
-__device__ __forceinline__ void wave_lds_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 // Candidates a wave stages per row: its lanes' windows of one row lie in
 // one sorted range of about 64 + 2 chords of particles when the wave sits in
 // one y/z column; a wider range (a wave across two columns) is walked from
@@ -308,6 +400,25 @@ __global__ __launch_bounds__(BLOCK) void density_kernel(StepArgs a) {
         const SceneConst& sc = a.scenes[ti.x];
         const uint32_t k = ti.y + threadIdx.x;
         if (k < sc.p_end) density_one(a, sc, k, rpos, rx, ry, rz, rcs);
+    });
+}
+
+// density_kernel with the rows staged in LDS (1.5 KB per wave).
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void density_staged_kernel(StepArgs a) {
+    if (a.halt && *a.halt) return;
+    __shared__ DensitySlot s_slot[BLOCK / 64];
+    const __amdgpu_buffer_rsrc_t rpos = buffer_of(a.pos_s, a.n);
+    const __amdgpu_buffer_rsrc_t rx = buffer_of(a.soa, a.n);
+    const __amdgpu_buffer_rsrc_t ry = buffer_of(a.soa + a.n, a.n);
+    const __amdgpu_buffer_rsrc_t rz = buffer_of(a.soa + 2 * (size_t)a.n, a.n);
+    const __amdgpu_buffer_rsrc_t rcs = buffer_of(a.cstart, a.ncells + 1u);
+    DensitySlot& slot = s_slot[threadIdx.x >> 6];
+    for_each_tile(a.ntiles, a.queue, [&](uint32_t t) {
+        const uint2 ti = a.tiles[t];
+        const SceneConst& sc = a.scenes[ti.x];
+        const uint32_t k = ti.y + threadIdx.x;
+        density_one_staged(a, sc, k, k < sc.p_end, rpos, rx, ry, rz, rcs, slot);
     });
 }
 
@@ -510,9 +621,13 @@ inline uint32_t blocks_for(size_t n, int b) { return (uint32_t)((n + b - 1) / b)
         default: hipLaunchKernelGGL(KERNEL<256>, dim3(grid), dim3(256), 0, st, __VA_ARGS__); break;  \
     }
 
-hipError_t launch_density(hipStream_t st, int block, uint32_t grid, const StepArgs& a) {
+hipError_t launch_density(hipStream_t st, int block, uint32_t grid, const StepArgs& a, int staged) {
     if (grid == 0) return hipSuccess;
-    SPH_DISPATCH_BLOCK(density_kernel, block, grid, st, a);
+    if (staged) {
+        SPH_DISPATCH_BLOCK(density_staged_kernel, block, grid, st, a);
+    } else {
+        SPH_DISPATCH_BLOCK(density_kernel, block, grid, st, a);
+    }
     return hipGetLastError();
 }
 

@@ -376,15 +376,16 @@ class Engine:
         self._chk(self._L.sph_get_tuning(self._h, _p(cfg)))
         return dict(density_block=int(cfg[0]), force_block=int(cfg[1]),
                     reorder_block=int(cfg[2]), sort_items=int(cfg[3]), persistent=int(cfg[4]) & 1,
-                    force_staged=(int(cfg[4]) >> 1) & 1)
+                    force_staged=(int(cfg[4]) >> 1) & 1, density_staged=(int(cfg[4]) >> 2) & 1)
 
     def set_tuning(self, density_block=256, force_block=256, reorder_block=256, sort_items=8, persistent=0,
-                   force_staged=0):
+                   force_staged=0, density_staged=0):
         """persistent: density and force as persistent blocks over tile
-        queues; force_staged: the force pass reads each row's candidates from
-        LDS.  Every choice gives the same results bit for bit."""
+        queues; force_staged / density_staged: that pass reads each row's
+        candidates from LDS.  Every choice gives the same results bit for
+        bit."""
         cfg = np.array([density_block, force_block, reorder_block, sort_items,
-                        (persistent & 1) | ((force_staged & 1) << 1)], np.int32)
+                        (persistent & 1) | ((force_staged & 1) << 1) | ((density_staged & 1) << 2)], np.int32)
         self._chk(self._L.sph_set_tuning(self._h, _p(cfg)))
 
     def set_sort_mode(self, mode: int):

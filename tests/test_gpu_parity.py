@@ -290,9 +290,10 @@ def test_clustered_cloud_parity(E):
 
 def test_rows_longer_than_the_mask_parity(E):
     """Rows of more than 32 candidates (their head re-tested in the force
-    pass), with the force pass gathering from global memory and with its rows
-    staged in LDS (rows up to 128 candidates staged, longer ones gathered);
-    the two forms give the same bits."""
+    pass), with the density and force passes reading candidates from global
+    memory and with their rows staged in LDS (rows up to 128 candidates
+    staged, longer ones read from global memory); the two forms give the
+    same bits."""
     params, pos = compressed_block_scene()
     for xsub in (1, 4):
         out = []
@@ -301,7 +302,7 @@ def test_rows_longer_than_the_mask_parity(E):
             e.set_particles(pos)
             e.set_cell_subdivision(xsub)
             e.make_ready()
-            e.set_tuning(256, 256, 256, 8, 0, staged)
+            e.set_tuning(256, 256, 256, 8, 0, staged, staged)
             d = step_and_compare(E, e, params, f"compressed block xsub {xsub} staged {staged}", vs_fp64=True)
             step_and_compare(E, e, params, f"compressed block xsub {xsub} staged {staged} step 2", vs_fp64=True)
             e.advance_steps(3)
@@ -330,15 +331,16 @@ def test_full_size_step_parity(E, name, steps, staged):
     some engine steps, one step from the identical state on the GPU and on
     the oracle, every particle: cell keys, permutation, cell table and
     neighbour sets bit-exact, density, acceleration and state within the
-    module's tolerances.  staged: the force pass reads each row's candidates
-    from LDS."""
+    module's tolerances.  staged: the density and force passes read each
+    row's candidates from LDS."""
     from simplefluid_amd import configs
     cfg = getattr(configs, name)()
     p = cfg["params"]
-    say(f"[{name}] {len(cfg['pos'])} particles, {steps} engine steps first, staged force {staged}")
+    say(f"[{name}] {len(cfg['pos'])} particles, {steps} engine steps first, staged passes {staged}")
     e = make_engine(E, p, cfg["pos"])
-    e.set_tuning(256, 256, 256, 8, 0, staged)
-    assert e.get_tuning()["force_staged"] == staged
+    e.set_tuning(256, 256, 256, 8, 0, staged, staged)
+    t = e.get_tuning()
+    assert t["force_staged"] == staged and t["density_staged"] == staged
     e.advance_steps(steps)
     step_and_compare(E, e, p, f"{name} full")
     e.close()
@@ -515,7 +517,8 @@ def test_runs_are_deterministic_and_tuning_invariant(E):
     cfg = configs.dambreak_65k(jitter=1e-6, seed=4)
     res = []
     for tune in ((256, 256, 256, 8), (256, 256, 256, 8), (64, 512, 128, 16), (512, 128, 64, 4, 1),
-                 (256, 256, 256, 8, 0, 1), (64, 64, 128, 16, 1, 1), (512, 512, 256, 8, 0, 1)):
+                 (256, 256, 256, 8, 0, 1), (64, 64, 128, 16, 1, 1), (512, 512, 256, 8, 0, 1),
+                 (256, 256, 256, 8, 0, 0, 1), (64, 512, 128, 16, 1, 1, 1), (512, 64, 256, 8, 0, 1, 1)):
         e = make_engine(E, cfg["params"], cfg["pos"])
         e.set_tuning(*tune)
         e.advance_steps(30)
@@ -533,7 +536,7 @@ def test_persistent_launch_matches_one_block_per_tile(E):
     cfg = configs.double_dambreak_1m()
     res = []
     for tune in ((64, 64, 256, 8, 0), (64, 64, 256, 8, 1), (512, 256, 256, 8, 1), (512, 64, 256, 8, 1, 1),
-                 (512, 512, 256, 8, 0, 1)):
+                 (512, 512, 256, 8, 0, 1), (64, 256, 256, 8, 1, 1, 1), (512, 256, 256, 8, 0, 0, 1)):
         e = make_engine(E, cfg["params"], cfg["pos"])
         e.set_tuning(*tune)
         e.advance_steps(8)
@@ -613,7 +616,7 @@ def test_tuning_changes_between_captured_frames(E):
     pos = E.scene_generate(E.SCENE_DAMBREAK, 24)
     res = []
     for plan in (((256, 256, 256, 8),), ((64, 128, 256, 8), (512, 64, 256, 8, 1), (128, 512, 256, 8)),
-                 ((64, 128, 256, 8, 0, 1), (512, 64, 256, 8, 1, 1), (128, 512, 256, 8))):
+                 ((64, 128, 256, 8, 0, 1), (512, 64, 256, 8, 1, 1, 1), (128, 512, 256, 8, 0, 0, 1))):
         e = make_engine(E, p, pos)
         frames = []
         for i in range(6):
@@ -1159,8 +1162,8 @@ def test_batched_with_empty_scenes(E):
 def test_batched_persistent_queues_match(E):
     """A batch of six ~100k-particle scenes (more 64-lane tiles than resident
     blocks, so the per-XCD tile queues are in use) gives the same bits with
-    persistent blocks, and with the force pass staging its rows in LDS, as
-    with one block per tile."""
+    persistent blocks, and with the density and force passes staging their
+    rows in LDS, as with one block per tile."""
     from simplefluid_amd import configs
     ens = configs.ensemble(nscenes=6, seed=12, nmin=90000, nmax=110000)
     params = [s["params"] for s in ens["scenes"]]
@@ -1171,7 +1174,7 @@ def test_batched_persistent_queues_match(E):
         b = E.BatchEngine(params, counts)
         b.set_particles(np.concatenate([s["pos"] for s in ens["scenes"]]))
         b.make_ready()
-        b.set_tuning(64, 64, 256, 8, persistent, staged)
+        b.set_tuning(64, 64, 256, 8, persistent, staged, staged)
         b.advance_steps(15)
         out.append((b.get_particles(), b.get_velocity()))
         b.close()
