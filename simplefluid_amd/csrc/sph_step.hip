@@ -139,9 +139,11 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Candidates a wave stages per row in the density pass, plus the three
-// slots a row's last, partial trip reads past its end (masked out).
-constexpr uint32_t kStageD = 128;
+// Candidates a wave stages per row in the density pass, plus the slots a
+// row's last, partial trip reads past its end (masked out).  With 124
+// floats per plane, a trip's reads of all three planes fit the 8-bit dword
+// offsets of ds_read2_b32 from one address (z at 248..251).
+constexpr uint32_t kStageD = 120;
 struct DensitySlot {
     float x[kStageD + 4];
     float y[kStageD + 4];
