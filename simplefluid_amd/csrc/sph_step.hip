@@ -103,19 +103,22 @@ __device__ __forceinline__ void density_one(const StepArgs& a, const SceneConst&
 }
 
 // Wave-wide minimum / maximum of one value per lane (all 64 lanes active):
-// DPP within each row of 16 lanes, then the four rows' results.
+// DPP within each row of 16 lanes, then the four rows' results.  The
+// permutations read no lane outside the row, so the "old" operand (the
+// operation's identity) is never used, and the compiler folds each move
+// into the min / max as its DPP source.
 __device__ __forceinline__ uint32_t row16_min(uint32_t v) {
-    v = min(v, (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0xB1, 0xF, 0xF, false));   // quad_perm 1,0,3,2
-    v = min(v, (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x4E, 0xF, 0xF, false));   // quad_perm 2,3,0,1
-    v = min(v, (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x141, 0xF, 0xF, false));  // row_half_mirror
-    v = min(v, (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x140, 0xF, 0xF, false));  // row_mirror
+    v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)v, 0xB1, 0xF, 0xF, false));   // quad_perm 1,0,3,2
+    v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)v, 0x4E, 0xF, 0xF, false));   // quad_perm 2,3,0,1
+    v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)v, 0x141, 0xF, 0xF, false));  // row_half_mirror
+    v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)v, 0x140, 0xF, 0xF, false));  // row_mirror
     return v;
 }
 __device__ __forceinline__ uint32_t row16_max(uint32_t v) {
-    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0xB1, 0xF, 0xF, false));
-    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x4E, 0xF, 0xF, false));
-    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x141, 0xF, 0xF, false));
-    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x140, 0xF, 0xF, false));
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false));
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false));
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x141, 0xF, 0xF, false));
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x140, 0xF, 0xF, false));
     return v;
 }
 __device__ __forceinline__ uint32_t wave_umin(uint32_t v) {
@@ -368,13 +371,16 @@ __device__ __forceinline__ uint32_t force_one_staged(const StepArgs& a, const Sc
                 slot.vel[e] = load4(rvel, (ulo + e) << 4);
             }
             wave_lds_sync();
+            // Entry j of the row sits at slot.pos[j - ulo], its velocity
+            // kStage records further on.
+            const float4* base = slot.pos - ulo;
             row_neighbors(
                 lo[row], hi[row], mk[row],
                 [&](uint32_t j) {
-                    const float4 pj = slot.pos[j - ulo];
+                    const float4 pj = base[j];
                     return j != k && dist2(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z) < sc.h2;
                 },
-                [&](uint32_t j) { force_pair(sc, pi, vi, slot.pos[j - ulo], slot.vel[j - ulo], f); });
+                [&](uint32_t j) { force_pair(sc, pi, vi, base[j], base[j + kStage], f); });
             wave_lds_sync();
         } else {
             row_neighbors(
