@@ -100,11 +100,12 @@ def plain(args):
     dom = max(("sort", "reorder", "density", "force"), key=lambda k: per[k])
     dom_bytes = B_ALG[dom] * n + (8.0 * occ if dom == "reorder" else 0.0)
     achieved = dom_bytes / (per[dom] * 1e-3) / 1e9
-    traffic = load_traffic(dom)
-    roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+    tuning = e.get_tuning()
+    traffic = load_traffic(dom, tuning)
+    roof = {"bound": "hbm", "kernel": profile_key(dom, tuning), "achieved": round(achieved, 2),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic}
-    valu = load_valu(dom)
+    valu = load_valu(dom, tuning)
     if valu:
         # Lane operations per launch (rocprofv3 SQ_INSTS_VALU x 64) over the
         # live mean duration: the issue-rate ceiling these kernels meet first.
@@ -184,25 +185,34 @@ def host_info():
             "numa_nodes": len(glob.glob("/sys/devices/system/node/node[0-9]*"))}
 
 
-def load_valu(kernel):
+def profile_key(kernel, tuning=None):
+    """Key of a pass's figures in the committed profile: density and force
+    have one entry per form (rows gathered from global memory, or staged in
+    LDS: "<pass>_staged")."""
+    if tuning and kernel in ("density", "force") and tuning.get(f"{kernel}_staged"):
+        return f"{kernel}_staged"
+    return kernel
+
+
+def load_valu(kernel, tuning=None):
     """VALU instructions per launch (wave-instructions) from the committed profile."""
     path = os.path.join(ROOT, "profiles", "valu.json")
     if not os.path.exists(path):
         return None
     try:
-        return json.load(open(path)).get(kernel, {}).get("valu_insts_per_launch")
+        return json.load(open(path)).get(profile_key(kernel, tuning), {}).get("valu_insts_per_launch")
     except Exception:
         return None
 
 
-def load_traffic(kernel):
+def load_traffic(kernel, tuning=None):
     """HBM bytes per launch from the committed rocprofv3 counter summary."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
         return None
     try:
         d = json.load(open(path))
-        return d.get(kernel, {}).get("bytes_per_launch")
+        return d.get(profile_key(kernel, tuning), {}).get("bytes_per_launch")
     except Exception:
         return None
 

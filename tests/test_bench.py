@@ -24,6 +24,13 @@ def test_profile_figures_for_the_roofline():
     for k in ("density", "force", "reorder", "sort"):
         assert bench.load_valu(k) > 0, k
         assert bench.load_traffic(k) > 0, k
+    # The staged forms of density and force have figures of their own, picked
+    # by the tuning the bench ran with.
+    for k in ("density", "force"):
+        staged = {f"{k}_staged": 1}
+        assert bench.profile_key(k, staged) == f"{k}_staged" and bench.profile_key(k, {}) == k
+        assert bench.load_valu(k, staged) > 0 and bench.load_valu(k, staged) != bench.load_valu(k)
+        assert bench.load_traffic(k, staged) > 0
     t = json.load(open(os.path.join(ROOT, "profiles", "traffic.json")))
     # HBM bytes per launch of the force pass: at least its state writes
     # (new position and velocity, 32 B per particle of the 1M workload).
