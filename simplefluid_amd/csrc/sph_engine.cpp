@@ -328,7 +328,7 @@ int enqueue_step(sph_handle* h, StepBufs& w, bool record_prof, uint32_t* frame) 
     sp.vel_in = h->vel.p;
     sp.id_in = h->id.p;
     sp.pos_out = w.pos_s.p;
-    sp.soa_out = w.soa.p;
+    sp.soa_out = h->tune.density_staged ? nullptr : w.soa.p;
     sp.halt = frame;
     sp.vel_out = w.vel_s.p;
     sp.id_out = w.id_s.p;

@@ -41,7 +41,8 @@ struct SortPlan {
     const float4* vel_in = nullptr;
     const uint32_t* id_in = nullptr;
     float4* pos_out = nullptr;
-    float* soa_out = nullptr;         // sorted x, y, z planes (3 N floats) for the density pass
+    float* soa_out = nullptr;         // sorted x, y, z planes (3 N floats) for the gathered density
+                                      // pass; null when the density pass is staged (it reads pos_out)
     float4* vel_out = nullptr;
     uint32_t* id_out = nullptr;
 };

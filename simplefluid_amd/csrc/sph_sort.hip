@@ -322,9 +322,11 @@ __global__ void cell_rank_kernel(uint32_t n, uint32_t C, const uint32_t* __restr
     skey[dst] = key;
     const float4 pk = pos_in[k];
     pos_out[dst] = pk;
-    soa[dst] = pk.x;
-    soa[n + dst] = pk.y;
-    soa[2 * (size_t)n + dst] = pk.z;
+    if (soa) {  // the planes, for the gathered density pass
+        soa[dst] = pk.x;
+        soa[n + dst] = pk.y;
+        soa[2 * (size_t)n + dst] = pk.z;
+    }
     vel_out[dst] = vel_in[k];
     id_out[dst] = id_in[k];
 }
@@ -498,9 +500,11 @@ __global__ void radix_reorder_kernel(uint32_t n, const uint32_t* __restrict__ pe
     }
     const float4 ps = pos_in[s];
     pos_out[k] = ps;
-    soa[k] = ps.x;
-    soa[n + k] = ps.y;
-    soa[2 * (size_t)n + k] = ps.z;
+    if (soa) {
+        soa[k] = ps.x;
+        soa[n + k] = ps.y;
+        soa[2 * (size_t)n + k] = ps.z;
+    }
     vel_out[k] = vel_in[s];
     id_out[k] = id_in[s];
 }

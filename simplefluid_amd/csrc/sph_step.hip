@@ -153,16 +153,31 @@ struct DensitySlot {
     float z[kStageD + 4];
 };
 
+// Four consecutive sorted positions as x, y, z quadruples (the gathered
+// trip's planes, read from the position records instead).
+typedef uint32_t u32x3 __attribute__((ext_vector_type(3)));
+__device__ __forceinline__ void load_quad(__amdgpu_buffer_rsrc_t rpos, uint32_t j, float4& X, float4& Y, float4& Z) {
+    // x, y, z of each record (12-byte loads: the w lane is not needed).
+    const u32x3 p0 = __builtin_amdgcn_raw_buffer_load_b96(rpos, (int)(j << 4), 0, 0);
+    const u32x3 p1 = __builtin_amdgcn_raw_buffer_load_b96(rpos, (int)((j + 1u) << 4), 0, 0);
+    const u32x3 p2 = __builtin_amdgcn_raw_buffer_load_b96(rpos, (int)((j + 2u) << 4), 0, 0);
+    const u32x3 p3 = __builtin_amdgcn_raw_buffer_load_b96(rpos, (int)((j + 3u) << 4), 0, 0);
+    X = make_float4(__uint_as_float(p0.x), __uint_as_float(p1.x), __uint_as_float(p2.x), __uint_as_float(p3.x));
+    Y = make_float4(__uint_as_float(p0.y), __uint_as_float(p1.y), __uint_as_float(p2.y), __uint_as_float(p3.y));
+    Z = make_float4(__uint_as_float(p0.z), __uint_as_float(p1.z), __uint_as_float(p2.z), __uint_as_float(p3.z));
+}
+
 // density_one with each row's candidates read from LDS: per row the wave
-// copies the x, y, z planes of the union of its lanes' windows into its slot
-// (coalesced), then every lane runs its own four-candidate trips over the
-// slot.  Same candidates, same order, same arithmetic: density_one's results
-// bit for bit.  Every lane of the wave calls it; inactive lanes have empty
-// windows and store nothing.
+// copies the positions of the union of its lanes' windows into its slot as
+// x, y, z planes (coalesced record loads), then every lane runs its own
+// four-candidate trips over the slot.  It reads the sorted position records
+// only, so the reorder pass need not write the planes for it.  Same
+// candidates, same order, same arithmetic: density_one's results bit for
+// bit.  Every lane of the wave calls it; inactive lanes have empty windows
+// and store nothing.
 __device__ __forceinline__ void density_one_staged(const StepArgs& a, const SceneConst& sc, uint32_t k, bool active,
-                                                   __amdgpu_buffer_rsrc_t rpos, __amdgpu_buffer_rsrc_t rx,
-                                                   __amdgpu_buffer_rsrc_t ry, __amdgpu_buffer_rsrc_t rz,
-                                                   __amdgpu_buffer_rsrc_t rcs, DensitySlot& slot) {
+                                                   __amdgpu_buffer_rsrc_t rpos, __amdgpu_buffer_rsrc_t rcs,
+                                                   DensitySlot& slot) {
     const float4 pi = load4(rpos, k << 4);  // zeros past the buffer end
     const RowGeom g = row_geom(sc, pi.x, pi.y, pi.z);
     uint32_t lo[kRows], hi[kRows];
@@ -183,10 +198,10 @@ __device__ __forceinline__ void density_one_staged(const StepArgs& a, const Scen
         const uint32_t uhi = wave_umax(has ? hi[row] : 0u);
         if (ulo < uhi && uhi - ulo <= kStageD) {
             for (uint32_t e = lane; e < uhi - ulo; e += 64u) {
-                const uint32_t off = (ulo + e) << 2;
-                slot.x[e] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rx, (int)off, 0, 0));
-                slot.y[e] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(ry, (int)off, 0, 0));
-                slot.z[e] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rz, (int)off, 0, 0));
+                const float4 p = load4(rpos, (ulo + e) << 4);
+                slot.x[e] = p.x;
+                slot.y[e] = p.y;
+                slot.z[e] = p.z;
             }
             wave_lds_sync();
             uint32_t o = lo[row] - ulo;
@@ -203,12 +218,19 @@ __device__ __forceinline__ void density_one_staged(const StepArgs& a, const Scen
                                     m);
             wave_lds_sync();
         } else if (has) {
-            uint32_t off = lo[row] << 2;
-            const uint32_t full_end = off + ((cnt & ~3u) << 2);
-            for (; off != full_end; off += 16u)
-                density_trip<true>(pi, h2, load4(rx, off), load4(ry, off), load4(rz, off), 4, sum, m);
+            // Past the buffer end the records read as zeros, as the planes do.
+            uint32_t j = lo[row];
+            const uint32_t full_end = j + (cnt & ~3u);
+            float4 X, Y, Z;
+            for (; j != full_end; j += 4u) {
+                load_quad(rpos, j, X, Y, Z);
+                density_trip<true>(pi, h2, X, Y, Z, 4, sum, m);
+            }
             const int rem = (int)(cnt & 3u);
-            if (rem > 0) density_trip<false>(pi, h2, load4(rx, off), load4(ry, off), load4(rz, off), rem, sum, m);
+            if (rem > 0) {
+                load_quad(rpos, j, X, Y, Z);
+                density_trip<false>(pi, h2, X, Y, Z, rem, sum, m);
+            }
         }
         if (active) a.mask[(size_t)row * a.n + k] = m;
     }
@@ -411,22 +433,20 @@ __global__ __launch_bounds__(BLOCK) void density_kernel(StepArgs a) {
     });
 }
 
-// density_kernel with the rows staged in LDS (1.5 KB per wave).
+// density_kernel with the rows staged in LDS (1.5 KB per wave).  Eight
+// waves per SIMD: the bound fits it in 59 VGPRs without spills (65 without).
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void density_staged_kernel(StepArgs a) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8, 8))) void density_staged_kernel(StepArgs a) {
     if (a.halt && *a.halt) return;
     __shared__ DensitySlot s_slot[BLOCK / 64];
     const __amdgpu_buffer_rsrc_t rpos = buffer_of(a.pos_s, a.n);
-    const __amdgpu_buffer_rsrc_t rx = buffer_of(a.soa, a.n);
-    const __amdgpu_buffer_rsrc_t ry = buffer_of(a.soa + a.n, a.n);
-    const __amdgpu_buffer_rsrc_t rz = buffer_of(a.soa + 2 * (size_t)a.n, a.n);
     const __amdgpu_buffer_rsrc_t rcs = buffer_of(a.cstart, a.ncells + 1u);
     DensitySlot& slot = s_slot[threadIdx.x >> 6];
     for_each_tile(a.ntiles, a.queue, [&](uint32_t t) {
         const uint2 ti = a.tiles[t];
         const SceneConst& sc = a.scenes[ti.x];
         const uint32_t k = ti.y + threadIdx.x;
-        density_one_staged(a, sc, k, k < sc.p_end, rpos, rx, ry, rz, rcs, slot);
+        density_one_staged(a, sc, k, k < sc.p_end, rpos, rcs, slot);
     });
 }
 
