@@ -198,11 +198,9 @@ __device__ __forceinline__ void row_neighbors(uint32_t lo, uint32_t hi, uint32_t
     if (span > 32u)
         for (uint32_t j = lo; j < jb; ++j)
             if (retest(j)) f(j);
-    while (m) {
-        const uint32_t z = (uint32_t)__builtin_clz(m);
-        m ^= 0x80000000u >> z;
-        f(jb + z);
-    }
+    // Bit 31 - i of m is candidate jb + i; reversed, the lowest set bit is
+    // the next candidate in ascending order, and r & (r - 1) clears it.
+    for (uint32_t r = __builtin_bitreverse32(m); r; r &= r - 1u) f(jb + (uint32_t)__builtin_ctz(r));
 }
 
 
