@@ -16,8 +16,7 @@ sph_handle::~sph_handle() {
     if (snap_done) (void)hipEventDestroy(snap_done);
     if (copy_stream) (void)hipStreamDestroy(copy_stream);
     if (host_pinned) (void)hipHostFree(host_pinned);
-    if (gexec) (void)hipGraphExecDestroy(gexec);
-    if (graph) (void)hipGraphDestroy(graph);
+    sph::invalidate_graph(this);
     for (auto e : prof_events) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
 }
@@ -144,10 +143,14 @@ int derive_scene_noalloc(const sph_params& p, SceneConst& sc, int xsub) {
 void invalidate_graph(sph_handle* h) {
     if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
     if (h->graph) (void)hipGraphDestroy(h->graph);
+    if (h->gexec_multi) (void)hipGraphExecDestroy(h->gexec_multi);
+    if (h->graph_multi) (void)hipGraphDestroy(h->graph_multi);
     if (h->gexec_frame) (void)hipGraphExecDestroy(h->gexec_frame);
     if (h->graph_frame) (void)hipGraphDestroy(h->graph_frame);
     h->gexec = nullptr;
     h->graph = nullptr;
+    h->gexec_multi = nullptr;
+    h->graph_multi = nullptr;
     h->gexec_frame = nullptr;
     h->graph_frame = nullptr;
 }
@@ -621,9 +624,15 @@ static int step_once(sph_handle* h) {
 }
 
 // Captures one steady step; gated: the device frame loop's step.
-static int capture_graph(sph_handle* h, bool gated = false) {
-    hipGraphExec_t& gx = gated ? h->gexec_frame : h->gexec;
-    hipGraph_t& gr = gated ? h->graph_frame : h->graph;
+// Steps in the multi-step graph.
+constexpr uint32_t kGraphSteps = 8;
+
+enum GraphKind { kGraphOne, kGraphMulti, kGraphFrame };
+
+static int capture_graph(sph_handle* h, GraphKind kind = kGraphOne) {
+    const bool gated = kind == kGraphFrame;
+    hipGraphExec_t& gx = gated ? h->gexec_frame : kind == kGraphMulti ? h->gexec_multi : h->gexec;
+    hipGraph_t& gr = gated ? h->graph_frame : kind == kGraphMulti ? h->graph_multi : h->graph;
     if (gx) (void)hipGraphExecDestroy(gx);
     if (gr) (void)hipGraphDestroy(gr);
     gx = nullptr;
@@ -639,7 +648,10 @@ static int capture_graph(sph_handle* h, bool gated = false) {
     (void)force_blocks_per_cu(h->tune.force_block, h->tune.force_staged);  // runtime query, outside capture
     hipError_t e = hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return hip_fail(h, e, "begin capture");
-    const int rc = enqueue_step(h, h->work, false, gated ? h->frame.p : nullptr);
+    int rc = SPH_OK;
+    const uint32_t steps = kind == kGraphMulti ? kGraphSteps : 1u;
+    for (uint32_t s = 0; s < steps && rc == SPH_OK; ++s)
+        rc = enqueue_step(h, h->work, false, gated ? h->frame.p : nullptr);
     hipGraph_t g = nullptr;
     e = hipStreamEndCapture(h->stream, &g);
     if (rc != SPH_OK) {
@@ -676,20 +688,24 @@ int sph_advance_steps(sph_handle* h, uint32_t k) {
     BudgetScope budget_scope(h ? &h->mem : nullptr);
     if (!h) return SPH_E_INVALID;
     if (!h->ready) return fail(h, SPH_E_NOT_READY, "call sph_make_ready first");
-    for (uint32_t i = 0; i < k; ++i) {
-        // The graph holds a steady-state step: keys from the last integrate,
+    for (uint32_t i = 0; i < k;) {
+        // The graphs hold steady-state steps: keys from the last integrate,
         // a cell table to clear sparsely.  Earlier steps run eagerly.
         const bool steady = h->keys_valid && !h->keep_acc;
         if (h->use_graph && steady && !h->prof) {
-            if (!h->gexec) {
-                int rc = capture_graph(h);
+            const bool multi = k - i >= kGraphSteps;
+            hipGraphExec_t& gx = multi ? h->gexec_multi : h->gexec;
+            if (!gx) {
+                const int rc = capture_graph(h, multi ? kGraphMulti : kGraphOne);
                 if (rc != SPH_OK) return rc;
             }
-            hipError_t e = hipGraphLaunch(h->gexec, h->stream);
+            const hipError_t e = hipGraphLaunch(gx, h->stream);
             if (e != hipSuccess) return hip_fail(h, e, "graph launch");
+            i += multi ? kGraphSteps : 1u;
         } else {
-            int rc = step_once(h);
+            const int rc = step_once(h);
             if (rc != SPH_OK) return rc;
+            ++i;
         }
     }
     return SPH_OK;
@@ -705,7 +721,7 @@ static int run_frame_on_device(sph_handle* h, float& frame_time, uint32_t& steps
     hipError_t e = launch_frame_reset(st, h->frame.p);
     if (e != hipSuccess) return hip_fail(h, e, "frame reset");
     if (!h->gexec_frame) {
-        const int rc = capture_graph(h, true);
+        const int rc = capture_graph(h, kGraphFrame);
         if (rc != SPH_OK) return rc;
     }
     uint32_t batch = h->frame_steps_hint ? h->frame_steps_hint + 1 : 16;
@@ -748,12 +764,27 @@ int sph_run_frame(sph_handle* h, float* frame_time_out, uint32_t* steps_out) {
         if (steps_out) *steps_out = steps;
         return SPH_OK;
     }
-    while (frame_time < 0.0333333333) {
-        float dt = 0.0f;
-        int rc = sph_advance_frame(h, &dt);
+    if (!h->cfl) {
+        // Fixed step: the frame's step count follows from the same float sum
+        // on the host alone, and the steps run as one call (multi-step graphs).
+        if (!h->ready) return fail(h, SPH_E_NOT_READY, "call sph_make_ready first");
+        const float dt = h->params[0].dt;
+        if (!(dt > 0.0f)) return fail(h, SPH_E_INVALID, "dt must be positive");
+        while (frame_time < 0.0333333333) {
+            frame_time += dt;
+            // A step too short to move the float sum would never end the frame.
+            if (++steps > (1u << 24)) return fail(h, SPH_E_INVALID, "dt is too small to reach a 1/30 s frame");
+        }
+        const int rc = sph_advance_steps(h, steps);
         if (rc != SPH_OK) return rc;
-        frame_time += dt;
-        ++steps;
+    } else {
+        while (frame_time < 0.0333333333) {
+            float dt = 0.0f;
+            int rc = sph_advance_frame(h, &dt);
+            if (rc != SPH_OK) return rc;
+            frame_time += dt;
+            ++steps;
+        }
     }
     if (frame_time_out) *frame_time_out = frame_time;
     if (steps_out) *steps_out = steps;

@@ -170,10 +170,13 @@ struct sph_handle {
     uint32_t snap_count = 0;               // particles when the pending snapshot began
     bool snap_pending = false;
 
-    // Graph of one step.
+    // Graph of one step, and of kGraphSteps steps in a row (replayed while at
+    // least that many steps remain: one launch instead of several).
     bool use_graph = true;
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec_multi = nullptr;
+    hipGraph_t graph_multi = nullptr;
     // Device frame loop (CFL): a step graph whose kernels skip once the
     // frame state says the frame is complete; frame holds that state.
     hipGraphExec_t gexec_frame = nullptr;

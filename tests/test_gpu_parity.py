@@ -480,7 +480,8 @@ def test_reference_scene_defaults_parity(E, scene):
 def test_graph_replay_equals_eager(E):
     """Graph replays, with the program's own eager work between them
     (downloads, snapshots, synchronisation), equal the eagerly launched step
-    after every replay."""
+    after every replay: the one-step graph, the eight-step graph (calls of
+    8, 9 and 17 steps) and mixes of both."""
     from simplefluid_amd import configs
     cfg = configs.dambreak_65k(jitter=1e-6, seed=3)
     g = make_engine(E, cfg["params"], cfg["pos"])
@@ -488,7 +489,7 @@ def test_graph_replay_equals_eager(E):
     g.set_use_graph(True)
     r.set_use_graph(False)
     for i in range(16):
-        k = 1 + i % 3
+        k = (1, 2, 3, 8, 9, 17)[i % 6]
         g.advance_steps(k)
         r.advance_steps(k)
         xg = g.get_particles()
@@ -618,6 +619,36 @@ def test_cfl_frames_on_device_match_the_host_loop(E):
     assert all(t >= np.float32(0.0333333333) for t, _ in res[True][0])
     assert len(set(k for _, k in res[True][0])) > 1   # the step length varies
     assert np.array_equal(res[True][1], res[False][1]) and np.array_equal(res[True][2], res[False][2])
+
+
+def test_fixed_step_frame_runs_its_steps_in_one_call(E):
+    """With a fixed step, a frame's step count follows from the float sum
+    of dt alone: run_frame gives the bits of the same number of single
+    steps, and a dt too small to move the sum is refused, not looped on."""
+    p = E.default_params(16)
+    pos = E.scene_generate(E.SCENE_DAMBREAK, 16)
+    a = make_engine(E, p, pos)
+    b = make_engine(E, p, pos)
+    t, steps = a.run_frame()
+    ft = np.float32(0.0)
+    n = 0
+    while ft < 0.0333333333:
+        ft = np.float32(ft + np.float32(p.dt))
+        n += 1
+    assert steps == n and t == float(ft)
+    for _ in range(n):
+        b.advance_steps(1)
+    assert np.array_equal(a.get_particles(), b.get_particles())
+    assert np.array_equal(a.get_velocity(), b.get_velocity())
+    a.close()
+    b.close()
+    q = E.default_params(16)
+    q.dt = 1e-12
+    c = make_engine(E, q, pos)
+    with pytest.raises(E.SPHError) as ei:
+        c.run_frame()
+    assert ei.value.code == E.SPH_E_INVALID and "too small" in str(ei.value)
+    c.close()
 
 
 def test_tuning_changes_between_captured_frames(E):
