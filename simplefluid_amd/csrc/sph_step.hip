@@ -7,7 +7,9 @@
 // Neighbour order is canonical: rows (dz, dy) = (-1..1, -1..1), dz outer; a
 // row is the contiguous sorted range of the x cells inside the support
 // sphere's chord at that row; ascending index.  Density sums and force sums
-// run in that order, one particle per lane.
+// run in that order, one particle per lane.  Density and force each come in
+// two forms with the same results: candidates read from global memory, or
+// each row's candidates first copied by the wave into LDS ("staged").
 #include <atomic>
 
 #include "sph_internal.hpp"
