@@ -484,6 +484,8 @@ def adjoint(args):
     vel0 = np.concatenate([s["vel"] for s in sc])
     b.set_particles(pos0, vel0)
     b.make_ready()
+    if not args.no_autotune:
+        b.autotune()  # launch shape of the forward passes (the state is restored)
     # Warm-up of both passes on the timed shape, so that neither timed region
     # allocates its checkpoints or tape.
     b.rollout_forward(rc["steps"], rc["checkpoint_every"])
