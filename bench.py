@@ -484,8 +484,7 @@ def adjoint(args):
     vel0 = np.concatenate([s["vel"] for s in sc])
     b.set_particles(pos0, vel0)
     b.make_ready()
-    if not args.no_autotune:
-        b.autotune()  # launch shape of the forward passes (the state is restored)
+    tune_leg(b, args)  # launch shape of the forward passes (the state is restored)
     # Warm-up of both passes on the timed shape, so that neither timed region
     # allocates its checkpoints or tape.
     b.rollout_forward(rc["steps"], rc["checkpoint_every"])
@@ -502,12 +501,14 @@ def adjoint(args):
     t0 = time.perf_counter()
     dB, dmu, dv0 = b.rollout_backward(g)
     tb = time.perf_counter() - t0
+    tuning = b.get_tuning()
     b.close()
     log(f"adjoint: forward {tf:.4f}s backward {tb:.4f}s ratio {tb / tf:.3f} loss {loss:.4e}")
     return {"scenes": len(sc), "particles": int(sum(counts)), "steps": rc["steps"],
             "checkpoint_every": rc["checkpoint_every"], "forward_s": round(tf, 5),
             "backward_s": round(tb, 5), "ratio": round(tb / tf, 3), "loss": loss,
-            "dL_dstiffness_scene0": float(dB[0]), "dL_dviscosity_scene0": float(dmu[0])}
+            "dL_dstiffness_scene0": float(dB[0]), "dL_dviscosity_scene0": float(dmu[0]),
+            "tuning": tuning}
 
 
 def calibrate_copy():

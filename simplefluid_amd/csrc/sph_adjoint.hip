@@ -24,6 +24,9 @@
 //   adj_force      starts Pbar_i from it (the term involves particle i alone)
 //   adj_density:   xbar_i,d += rhobar_i rho0 (wall_rho'(u_lo) - wall_rho'(u_hi)) / h
 // Each kernel reads neighbour values written by the previous kernel only.
+// adj_force and adj_density come in two forms with identical results: the
+// neighbours' records gathered from global memory, or each row's candidates
+// staged in LDS by the wave first (the forward passes' staged form).
 #include "sph_internal.hpp"
 
 namespace sph {
@@ -107,40 +110,48 @@ __global__ void adj_integrate_kernel(AdjArgs a) {
     }
 }
 
-// One particle of the adjoint force pass; returns its (dL/dB, dL/dmu) terms.
-__device__ __forceinline__ double2 adj_force_one(const AdjArgs& a, const SceneConst& sc, uint32_t k) {
-    const float4 pi = a.pos_s[k], vi = a.vel_s[k], abi = a.abar[k];
-    const float Pi = pi.w, qi = vi.w;  // p/rho^2 and 1/rho, stored by the density pass
-    float gx = 0, gy = 0, gz = 0, hx = 0, hy = 0, hz = 0, Pb = abi.w, qb = 0, Ax = 0, Ay = 0, Az = 0;
-    for_each_neighbor(a, sc, k, pi, [&](uint32_t j) {
-        const float4 pj = a.pos_s[j], vj = a.vel_s[j], abj = a.abar[j];
-        const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-        const float d2 = dist2(rx, ry, rz);
-        const float inv_r = d2 > 0.0f ? __builtin_amdgcn_rsqf(d2) : 0.0f;
-        const float dd = d2 * inv_r;
-        const float hr = sc.h - dd;
-        const float tt = sc.h2 - d2;
-        const float phi = hr * hr * inv_r;
-        const float dphi_r = -tt * inv_r * inv_r * inv_r;
-        const float dax = abi.x - abj.x, day = abi.y - abj.y, daz = abi.z - abj.z;
-        const float dvx = vj.x - vi.x, dvy = vj.y - vi.y, dvz = vj.z - vi.z;
-        const float PP = sc.cP * (Pi + pj.w);
-        const float dar = dax * rx + day * ry + daz * rz;
-        const float dav = dax * dvx + day * dvy + daz * dvz;
-        const float cq = sc.cV * qi * vj.w;
-        const float coefr = PP * dphi_r * dar - cq * inv_r * dav;
-        gx += PP * phi * dax + coefr * rx;
-        gy += PP * phi * day + coefr * ry;
-        gz += PP * phi * daz + coefr * rz;
-        hx -= cq * hr * dax;
-        hy -= cq * hr * day;
-        hz -= cq * hr * daz;
-        Pb += sc.cP * phi * dar;
-        qb += sc.cV * vj.w * hr * dav;
-        Ax += hr * vj.w * dvx;
-        Ay += hr * vj.w * dvy;
-        Az += hr * vj.w * dvz;
-    });
+// Per-particle sums of the adjoint force pass.
+struct AdjForceAcc {
+    float gx, gy, gz, hx, hy, hz, Pb, qb, Ax, Ay, Az;
+};
+
+// One neighbour's terms of the adjoint force pass (pos.w = p/rho^2, vel.w = 1/rho).
+__device__ __forceinline__ void adj_force_pair(const SceneConst& sc, const float4& pi, const float4& vi,
+                                               const float4& abi, const float4& pj, const float4& vj,
+                                               const float4& abj, AdjForceAcc& c) {
+    const float Pi = pi.w, qi = vi.w;
+    const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+    const float d2 = dist2(rx, ry, rz);
+    const float inv_r = d2 > 0.0f ? __builtin_amdgcn_rsqf(d2) : 0.0f;
+    const float dd = d2 * inv_r;
+    const float hr = sc.h - dd;
+    const float tt = sc.h2 - d2;
+    const float phi = hr * hr * inv_r;
+    const float dphi_r = -tt * inv_r * inv_r * inv_r;
+    const float dax = abi.x - abj.x, day = abi.y - abj.y, daz = abi.z - abj.z;
+    const float dvx = vj.x - vi.x, dvy = vj.y - vi.y, dvz = vj.z - vi.z;
+    const float PP = sc.cP * (Pi + pj.w);
+    const float dar = dax * rx + day * ry + daz * rz;
+    const float dav = dax * dvx + day * dvy + daz * dvz;
+    const float cq = sc.cV * qi * vj.w;
+    const float coefr = PP * dphi_r * dar - cq * inv_r * dav;
+    c.gx += PP * phi * dax + coefr * rx;
+    c.gy += PP * phi * day + coefr * ry;
+    c.gz += PP * phi * daz + coefr * rz;
+    c.hx -= cq * hr * dax;
+    c.hy -= cq * hr * day;
+    c.hz -= cq * hr * daz;
+    c.Pb += sc.cP * phi * dar;
+    c.qb += sc.cV * vj.w * hr * dav;
+    c.Ax += hr * vj.w * dvx;
+    c.Ay += hr * vj.w * dvy;
+    c.Az += hr * vj.w * dvz;
+}
+
+// The EOS reverse and the stores of particle k; returns its (dL/dB, dL/dmu) terms.
+__device__ __forceinline__ double2 adj_force_finish(const AdjArgs& a, const SceneConst& sc, uint32_t k,
+                                                    const float4& vi, const float4& abi, const AdjForceAcc& c) {
+    const float qi = vi.w;
     // EOS derivatives (clamped Tait, exponent 7).
     const float rho = a.rho[k];
     const float x = rho * sc.inv_rho0;
@@ -150,15 +161,107 @@ __device__ __forceinline__ double2 adj_force_one(const AdjArgs& a, const SceneCo
     const float p = praw * sgn;
     const float dp_drho = sgn * sc.B * 7.0f * x6 * sc.inv_rho0;
     const float dp_dB = sgn * (x7 - 1.0f);
-    a.rhobar[k] = Pb * (dp_drho * qi * qi - 2.0f * p * qi * qi * qi) - qb * qi * qi;
-    const float bpart = Pb * qi * qi * dp_dB;
-    const float mpart = sc.cP * qi * (abi.x * Ax + abi.y * Ay + abi.z * Az);
+    a.rhobar[k] = c.Pb * (dp_drho * qi * qi - 2.0f * p * qi * qi * qi) - c.qb * qi * qi;
+    const float bpart = c.Pb * qi * qi * dp_dB;
+    const float mpart = sc.cP * qi * (abi.x * c.Ax + abi.y * c.Ay + abi.z * c.Az);
     float4 xs = a.xs[k], vs = a.vs[k];
-    xs.x += gx; xs.y += gy; xs.z += gz;
-    vs.x += hx; vs.y += hy; vs.z += hz;
+    xs.x += c.gx; xs.y += c.gy; xs.z += c.gz;
+    vs.x += c.hx; vs.y += c.hy; vs.z += c.hz;
     a.xs[k] = xs;
     a.vs[k] = vs;
     return make_double2((double)bpart, (double)mpart);
+}
+
+// One particle of the adjoint force pass; returns its (dL/dB, dL/dmu) terms.
+__device__ __forceinline__ double2 adj_force_one(const AdjArgs& a, const SceneConst& sc, uint32_t k) {
+    const float4 pi = a.pos_s[k], vi = a.vel_s[k], abi = a.abar[k];
+    AdjForceAcc c = {0, 0, 0, 0, 0, 0, abi.w, 0, 0, 0, 0};
+    for_each_neighbor(a, sc, k, pi, [&](uint32_t j) {
+        adj_force_pair(sc, pi, vi, abi, a.pos_s[j], a.vel_s[j], a.abar[j], c);
+    });
+    return adj_force_finish(a, sc, k, vi, abi, c);
+}
+
+// Staged forms of the adjoint force and density passes: the forward passes'
+// staging applied to the reverse walk.  All nine windows and masks are formed
+// first; per row the wave copies the union of its lanes' windows into its LDS
+// slot (coalesced) and every lane walks its own mask over the slot; a wider
+// union is read from global memory.  Same neighbours, same order, same
+// arithmetic: the gathered kernels' results bit for bit.  Every lane of the
+// wave takes part; inactive lanes have empty windows and store nothing.
+constexpr uint32_t kStageA = 96;
+struct AdjForceSlot {
+    float4 pos[kStageA];
+    float4 vel[kStageA];
+    float4 abar[kStageA];
+};
+struct AdjDensitySlot {
+    float4 pos[kStageA];
+    float rb[kStageA];
+};
+
+struct AdjRows {
+    uint32_t lo[kRows], hi[kRows], mk[kRows];
+};
+
+__device__ __forceinline__ void adj_rows(const AdjArgs& a, const SceneConst& sc, uint32_t k, bool active,
+                                         const float4& pi, AdjRows& r) {
+    const __amdgpu_buffer_rsrc_t rcs = buffer_of(a.cstart, a.ncells + 1u);
+    const RowGeom g = row_geom(sc, pi.x, pi.y, pi.z);
+#pragma unroll
+    for (int row = 0; row < kRows; ++row) {
+        row_window(rcs, sc, g, row % 3 - 1, row / 3 - 1, r.lo[row], r.hi[row]);
+        r.mk[row] = active ? a.mask[(size_t)row * a.n + k] : 0u;
+        if (!active) r.lo[row] = r.hi[row] = 0u;
+    }
+    const uint32_t b = mask_bit(r.lo[4], r.hi[4], k);
+    if (active && b < (uint32_t)kMaskBits) r.mk[4] &= ~(1u << b);
+}
+
+// kv: a particle of the tile (the lane's own when active) whose values the
+// inactive lanes load in place of theirs.
+__device__ __forceinline__ double2 adj_force_one_staged(const AdjArgs& a, const SceneConst& sc, uint32_t k,
+                                                        bool active, uint32_t kv, AdjForceSlot& slot) {
+    const float4 pi = a.pos_s[kv], vi = a.vel_s[kv], abi = a.abar[kv];
+    AdjRows r;
+    adj_rows(a, sc, k, active, pi, r);
+    AdjForceAcc c = {0, 0, 0, 0, 0, 0, abi.w, 0, 0, 0, 0};
+    const uint32_t lane = threadIdx.x & 63u;
+#pragma unroll
+    for (int row = 0; row < kRows; ++row) {
+        const bool has = r.hi[row] > r.lo[row];
+        const uint32_t ulo = wave_umin(has ? r.lo[row] : 0xffffffffu);
+        const uint32_t uhi = wave_umax(has ? r.hi[row] : 0u);
+        if (ulo >= uhi) continue;
+        if (uhi - ulo <= kStageA) {
+            for (uint32_t e = lane; e < uhi - ulo; e += 64u) {
+                slot.pos[e] = a.pos_s[ulo + e];
+                slot.vel[e] = a.vel_s[ulo + e];
+                slot.abar[e] = a.abar[ulo + e];
+            }
+            wave_lds_sync();
+            row_neighbors(
+                r.lo[row], r.hi[row], r.mk[row],
+                [&](uint32_t j) {
+                    const float4 pj = slot.pos[j - ulo];
+                    return j != k && dist2(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z) < sc.h2;
+                },
+                [&](uint32_t j) {
+                    const uint32_t e = j - ulo;
+                    adj_force_pair(sc, pi, vi, abi, slot.pos[e], slot.vel[e], slot.abar[e], c);
+                });
+            wave_lds_sync();
+        } else {
+            row_neighbors(
+                r.lo[row], r.hi[row], r.mk[row],
+                [&](uint32_t j) {
+                    const float4 pj = a.pos_s[j];
+                    return j != k && dist2(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z) < sc.h2;
+                },
+                [&](uint32_t j) { adj_force_pair(sc, pi, vi, abi, a.pos_s[j], a.vel_s[j], a.abar[j], c); });
+        }
+    }
+    return active ? adj_force_finish(a, sc, k, vi, abi, c) : make_double2(0.0, 0.0);
 }
 
 // Eight waves per SIMD (64 VGPRs, a 36-byte spill): the backward pass was
@@ -195,6 +298,125 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
     }
 }
 
+// adj_force_kernel with the rows staged in LDS (4.5 KB per wave).
+__global__ __launch_bounds__(256) void adj_force_staged_kernel(AdjArgs a) {
+    uint32_t t0, t1;
+    tile_span(a.ntiles, t0, t1);
+    __shared__ double2 red[256];
+    __shared__ AdjForceSlot s_slot[256 / 64];
+    AdjForceSlot& slot = s_slot[threadIdx.x >> 6];
+    for (uint32_t t = t0; t < t1; ++t) {
+        const uint2 ti = a.tiles[t];
+        const SceneConst& sc = a.scenes[ti.x];
+        const uint32_t k = ti.y + threadIdx.x;
+        const bool active = k < sc.p_end;
+        const double2 part = adj_force_one_staged(a, sc, k, active, active ? k : ti.y, slot);
+        red[threadIdx.x] = part;
+        __syncthreads();
+        for (uint32_t o = blockDim.x >> 1; o > 0; o >>= 1) {
+            if (threadIdx.x < o) {
+                red[threadIdx.x].x += red[threadIdx.x + o].x;
+                red[threadIdx.x].y += red[threadIdx.x + o].y;
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            double2 acc = a.tile_acc[t];
+            acc.x += red[0].x;
+            acc.y += red[0].y;
+            a.tile_acc[t] = acc;
+        }
+        __syncthreads();
+    }
+}
+
+// One neighbour's term of the adjoint density pass.
+__device__ __forceinline__ void adj_density_pair(const SceneConst& sc, const float4& pi, float rbi, const float4& pj,
+                                                 float rbj, float& gx, float& gy, float& gz) {
+    const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+    const float tt = sc.h2 - dist2(rx, ry, rz);
+    const float s = -6.0f * sc.cD * tt * tt * (rbi + rbj);
+    gx += s * rx;
+    gy += s * ry;
+    gz += s * rz;
+}
+
+// The walls' density reverse, and the adjoints of particle k scattered back
+// to the step's input order.
+__device__ __forceinline__ void adj_density_finish(const AdjArgs& a, const SceneConst& sc, uint32_t k,
+                                                   const float4& pi, float rbi, float gx, float gy, float gz) {
+    const float pk[3] = {pi.x, pi.y, pi.z};
+    float gw[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const float ul = (pk[d] - sc.bmin[d]) * sc.inv_h, uh = (sc.wmax[d] - pk[d]) * sc.inv_h;
+        gw[d] = (rbi * sc.rho0) * (sc.inv_h * (wall_rho_du(ul) - wall_rho_du(uh)));
+    }
+    gx += gw[0];
+    gy += gw[1];
+    gz += gw[2];
+    float4 xs = a.xs[k];
+    xs.x += gx; xs.y += gy; xs.z += gz;
+    const uint32_t src = a.perm[k];
+    a.xbar[src] = xs;
+    a.vbar[src] = a.vs[k];
+}
+
+// adj_density_kernel with the rows staged in LDS (1.9 KB per wave).
+__global__ __launch_bounds__(256) void adj_density_staged_kernel(AdjArgs a) {
+    uint32_t t0, t1;
+    tile_span(a.ntiles, t0, t1);
+    __shared__ AdjDensitySlot s_slot[256 / 64];
+    AdjDensitySlot& slot = s_slot[threadIdx.x >> 6];
+    const uint32_t lane = threadIdx.x & 63u;
+    for (uint32_t t = t0; t < t1; ++t) {
+        const uint2 ti = a.tiles[t];
+        const SceneConst& sc = a.scenes[ti.x];
+        const uint32_t k = ti.y + threadIdx.x;
+        const bool active = k < sc.p_end;
+        const uint32_t kv = active ? k : ti.y;
+        const float4 pi = a.pos_s[kv];
+        const float rbi = a.rhobar[kv];
+        AdjRows r;
+        adj_rows(a, sc, k, active, pi, r);
+        float gx = 0, gy = 0, gz = 0;
+#pragma unroll
+        for (int row = 0; row < kRows; ++row) {
+            const bool has = r.hi[row] > r.lo[row];
+            const uint32_t ulo = wave_umin(has ? r.lo[row] : 0xffffffffu);
+            const uint32_t uhi = wave_umax(has ? r.hi[row] : 0u);
+            if (ulo >= uhi) continue;
+            if (uhi - ulo <= kStageA) {
+                for (uint32_t e = lane; e < uhi - ulo; e += 64u) {
+                    slot.pos[e] = a.pos_s[ulo + e];
+                    slot.rb[e] = a.rhobar[ulo + e];
+                }
+                wave_lds_sync();
+                row_neighbors(
+                    r.lo[row], r.hi[row], r.mk[row],
+                    [&](uint32_t j) {
+                        const float4 pj = slot.pos[j - ulo];
+                        return j != k && dist2(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z) < sc.h2;
+                    },
+                    [&](uint32_t j) {
+                        const uint32_t e = j - ulo;
+                        adj_density_pair(sc, pi, rbi, slot.pos[e], slot.rb[e], gx, gy, gz);
+                    });
+                wave_lds_sync();
+            } else {
+                row_neighbors(
+                    r.lo[row], r.hi[row], r.mk[row],
+                    [&](uint32_t j) {
+                        const float4 pj = a.pos_s[j];
+                        return j != k && dist2(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z) < sc.h2;
+                    },
+                    [&](uint32_t j) { adj_density_pair(sc, pi, rbi, a.pos_s[j], a.rhobar[j], gx, gy, gz); });
+            }
+        }
+        if (active) adj_density_finish(a, sc, k, pi, rbi, gx, gy, gz);
+    }
+}
+
 __global__ void adj_density_kernel(AdjArgs a) {
     uint32_t t0, t1;
     tile_span(a.ntiles, t0, t1);
@@ -207,31 +429,9 @@ __global__ void adj_density_kernel(AdjArgs a) {
         const float rbi = a.rhobar[k];
         float gx = 0, gy = 0, gz = 0;
         for_each_neighbor(a, sc, k, pi, [&](uint32_t j) {
-            const float4 pj = a.pos_s[j];
-            const float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-            const float tt = sc.h2 - dist2(rx, ry, rz);
-            const float s = -6.0f * sc.cD * tt * tt * (rbi + a.rhobar[j]);
-            gx += s * rx;
-            gy += s * ry;
-            gz += s * rz;
+            adj_density_pair(sc, pi, rbi, a.pos_s[j], a.rhobar[j], gx, gy, gz);
         });
-        {
-            const float pk[3] = {pi.x, pi.y, pi.z};
-            float gw[3];
-#pragma unroll
-            for (int d = 0; d < 3; ++d) {
-                const float ul = (pk[d] - sc.bmin[d]) * sc.inv_h, uh = (sc.wmax[d] - pk[d]) * sc.inv_h;
-                gw[d] = (rbi * sc.rho0) * (sc.inv_h * (wall_rho_du(ul) - wall_rho_du(uh)));
-            }
-            gx += gw[0];
-            gy += gw[1];
-            gz += gw[2];
-        }
-        float4 xs = a.xs[k];
-        xs.x += gx; xs.y += gy; xs.z += gz;
-        const uint32_t src = a.perm[k];
-        a.xbar[src] = xs;
-        a.vbar[src] = a.vs[k];
+        adj_density_finish(a, sc, k, pi, rbi, gx, gy, gz);
     }
 }
 
@@ -281,17 +481,25 @@ hipError_t launch_adj_integrate(hipStream_t st, int block, uint32_t grid, const 
     return hipGetLastError();
 }
 
-hipError_t launch_adj_force(hipStream_t st, int block, uint32_t grid, const AdjArgs& a) {
+hipError_t launch_adj_force(hipStream_t st, int block, uint32_t grid, const AdjArgs& a, int staged) {
     if (grid == 0) return hipSuccess;
-    // __launch_bounds__(256), and the per-tile tree wants a power of two.
-    if (block > 256 || (block & (block - 1))) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(adj_force_kernel, dim3(grid), dim3(block), 0, st, a);
+    // __launch_bounds__(256), and the per-tile tree wants a power of two;
+    // the staged form also wants whole waves.
+    if (block > 256 || (block & (block - 1)) || (staged && block < 64)) return hipErrorInvalidValue;
+    if (staged)
+        hipLaunchKernelGGL(adj_force_staged_kernel, dim3(grid), dim3(block), 0, st, a);
+    else
+        hipLaunchKernelGGL(adj_force_kernel, dim3(grid), dim3(block), 0, st, a);
     return hipGetLastError();
 }
 
-hipError_t launch_adj_density(hipStream_t st, int block, uint32_t grid, const AdjArgs& a) {
+hipError_t launch_adj_density(hipStream_t st, int block, uint32_t grid, const AdjArgs& a, int staged) {
     if (grid == 0) return hipSuccess;
-    hipLaunchKernelGGL(adj_density_kernel, dim3(grid), dim3(block), 0, st, a);
+    if (staged && (block > 256 || block % 64)) return hipErrorInvalidValue;
+    if (staged)
+        hipLaunchKernelGGL(adj_density_staged_kernel, dim3(grid), dim3(block), 0, st, a);
+    else
+        hipLaunchKernelGGL(adj_density_kernel, dim3(grid), dim3(block), 0, st, a);
     return hipGetLastError();
 }
 

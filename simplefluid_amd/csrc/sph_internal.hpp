@@ -147,8 +147,9 @@ struct AdjArgs {
 };
 
 hipError_t launch_adj_integrate(hipStream_t st, int block, uint32_t grid, const AdjArgs& a);
-hipError_t launch_adj_force(hipStream_t st, int block, uint32_t grid, const AdjArgs& a);
-hipError_t launch_adj_density(hipStream_t st, int block, uint32_t grid, const AdjArgs& a);
+// staged: each row's candidates read from LDS (same results; whole waves only).
+hipError_t launch_adj_force(hipStream_t st, int block, uint32_t grid, const AdjArgs& a, int staged = 0);
+hipError_t launch_adj_density(hipStream_t st, int block, uint32_t grid, const AdjArgs& a, int staged = 0);
 // Per-scene totals of the tile accumulators in a fixed order: scene s sums
 // tiles [tile_first[s], tile_first[s + 1]) into acc_a[s], acc_b[s].
 hipError_t launch_scene_tile_sums(hipStream_t st, uint32_t nscenes, const uint32_t* tile_first,

@@ -157,8 +157,9 @@ int sph_rollout_backward(sph_handle* h, const float* dL_dx_final, float* dL_dB, 
             a.rhobar = R.rhobar.p; a.tile_acc = R.tile_acc.p;
             a.scenes = h->sc_d.p; a.tiles = tiles; a.ntiles = ntiles; a.n = (uint32_t)n; a.ncells = h->ncells;
             chk(launch_adj_integrate(st, 256, grid, a));
-            chk(launch_adj_force(st, 256, grid, a));
-            chk(launch_adj_density(st, 256, grid, a));
+            // The reverse walk takes the forward force pass's form.
+            chk(launch_adj_force(st, 256, grid, a, h->tune.force_staged));
+            chk(launch_adj_density(st, 256, grid, a, h->tune.force_staged));
             if (e != hipSuccess) return hip_fail(h, e, "adjoint launch");
         }
     }

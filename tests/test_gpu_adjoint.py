@@ -23,12 +23,14 @@ def case(n3=(16, 16, 16), seed=0, B=50.0, mu=0.01, tscale=0.003):
     return p, pos, vel, target
 
 
-def gpu_grads(p, pos, vel, target, steps, every, xsub=0):
+def gpu_grads(p, pos, vel, target, steps, every, xsub=0, tuning=None):
     from simplefluid_amd import engine
     e = engine.Engine(p)
     e.set_cell_subdivision(xsub)
     e.set_particles(pos, vel)
     e.make_ready()
+    if tuning is not None:
+        e.set_tuning(*tuning)
     e.rollout_forward(steps, every)
     xT = e.get_particles().astype(np.float64)
     n = len(pos)
@@ -120,6 +122,33 @@ def test_adjoint_with_rows_longer_than_the_mask():
         assert abs(dB - oB) <= GRAD_RTOL * abs(oB)
         assert abs(dmu - omu) <= GRAD_RTOL * abs(omu)
         assert np.linalg.norm(dv0 - odv0) <= GRAD_RTOL * np.linalg.norm(odv0)
+
+
+def test_staged_adjoint_equals_gathered():
+    """The adjoint passes with each row's candidates staged in LDS (the form
+    taken with the staged forward force pass) give the gathered passes'
+    gradients bit for bit: on a dam-break block, and on a compressed block
+    whose rows exceed the mask and, at x subdivision 1, the staging slot."""
+    from simplefluid_amd import configs, engine
+    R = configs.R
+    rng = np.random.default_rng(22)
+    a = configs.lattice_block((8, 12, 8), (0.1, 0.0, 0.1))
+    i, j, k = np.meshgrid(np.arange(10), np.arange(12), np.arange(10), indexing="ij")
+    b = np.array([0.1 + 17 * R, R, 0.1 + R]) + 1.25 * R * np.stack([i.ravel(), j.ravel(), k.ravel()], 1)
+    pos = np.concatenate([a, b.astype(np.float32)])
+    vel = rng.normal(0, 0.1, pos.shape).astype(np.float32)
+    target = (pos + rng.normal(0, 0.003, pos.shape)).astype(np.float64)
+    p = engine.make_params(pressure_stiffness=50.0, viscosity=0.01, dt=1e-4)
+    cases = [(p, pos, vel, target, 6, 3, 1), (p, pos, vel, target, 6, 3, 4)]
+    p2, pos2, vel2, target2 = case(n3=(20, 24, 16), seed=5, B=200.0, mu=0.02)
+    cases.append((p2, pos2, vel2, target2, 10, 5, 0))
+    for p_, x, v, t, steps, every, xsub in cases:
+        say(f"staged vs gathered adjoint: n {len(x)} steps {steps} xsub {xsub}")
+        gat = gpu_grads(p_, x, v, t, steps, every, xsub=xsub, tuning=(256, 256, 256, 8, 0, 0, 0))
+        stg = gpu_grads(p_, x, v, t, steps, every, xsub=xsub, tuning=(256, 256, 256, 8, 0, 1, 1))
+        assert stg[1] == gat[1] and stg[2] == gat[2]
+        assert np.array_equal(stg[3], gat[3])
+        assert np.array_equal(stg[4], gat[4])
 
 
 def test_adjoint_attractive_pressure_and_zero_viscosity():
