@@ -69,8 +69,8 @@ __global__ void adj_integrate_kernel(AdjArgs a) {
     uint32_t t0, t1;
     tile_span(a.ntiles, t0, t1);
     for (uint32_t t = t0; t < t1; ++t) {
-        const uint2 ti = a.tiles[t];
-        const SceneConst& sc = a.scenes[ti.x];
+        const uint2 ti = tile_at(a.tiles, t);
+        const SceneConst sc = scene_at(a.scenes, ti.x);
         const uint32_t k = ti.y + threadIdx.x;
         if (k >= sc.p_end) continue;
         const float4 x0 = a.pos_s[k], v0 = a.vel_s[k], ac = a.acc[k];
@@ -240,7 +240,8 @@ __device__ __forceinline__ double2 adj_force_one_staged(const AdjArgs& a, const 
                 slot.abar[e] = a.abar[ulo + e];
             }
             wave_lds_sync();
-            row_neighbors(
+            const uint32_t e0 = row_first(r.lo[row], r.hi[row]) - ulo;
+            row_neighbors_from(
                 r.lo[row], r.hi[row], r.mk[row],
                 [&](uint32_t j) {
                     const float4 pj = slot.pos[j - ulo];
@@ -249,6 +250,9 @@ __device__ __forceinline__ double2 adj_force_one_staged(const AdjArgs& a, const 
                 [&](uint32_t j) {
                     const uint32_t e = j - ulo;
                     adj_force_pair(sc, pi, vi, abi, slot.pos[e], slot.vel[e], slot.abar[e], c);
+                },
+                [&](uint32_t i) {
+                    adj_force_pair(sc, pi, vi, abi, slot.pos[e0 + i], slot.vel[e0 + i], slot.abar[e0 + i], c);
                 });
             wave_lds_sync();
         } else {
@@ -274,8 +278,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
     // tree and added to the tile's running sum: deterministic, no atomics.
     __shared__ double2 red[256];
     for (uint32_t t = t0; t < t1; ++t) {
-        const uint2 ti = a.tiles[t];
-        const SceneConst& sc = a.scenes[ti.x];
+        const uint2 ti = tile_at(a.tiles, t);
+        const SceneConst sc = scene_at(a.scenes, ti.x);
         const uint32_t k = ti.y + threadIdx.x;
         double2 part = make_double2(0.0, 0.0);
         if (k < sc.p_end) part = adj_force_one(a, sc, k);
@@ -306,8 +310,8 @@ __global__ __launch_bounds__(256) void adj_force_staged_kernel(AdjArgs a) {
     __shared__ AdjForceSlot s_slot[256 / 64];
     AdjForceSlot& slot = s_slot[threadIdx.x >> 6];
     for (uint32_t t = t0; t < t1; ++t) {
-        const uint2 ti = a.tiles[t];
-        const SceneConst& sc = a.scenes[ti.x];
+        const uint2 ti = tile_at(a.tiles, t);
+        const SceneConst sc = scene_at(a.scenes, ti.x);
         const uint32_t k = ti.y + threadIdx.x;
         const bool active = k < sc.p_end;
         const double2 part = adj_force_one_staged(a, sc, k, active, active ? k : ti.y, slot);
@@ -370,8 +374,8 @@ __global__ __launch_bounds__(256) void adj_density_staged_kernel(AdjArgs a) {
     AdjDensitySlot& slot = s_slot[threadIdx.x >> 6];
     const uint32_t lane = threadIdx.x & 63u;
     for (uint32_t t = t0; t < t1; ++t) {
-        const uint2 ti = a.tiles[t];
-        const SceneConst& sc = a.scenes[ti.x];
+        const uint2 ti = tile_at(a.tiles, t);
+        const SceneConst sc = scene_at(a.scenes, ti.x);
         const uint32_t k = ti.y + threadIdx.x;
         const bool active = k < sc.p_end;
         const uint32_t kv = active ? k : ti.y;
@@ -392,7 +396,8 @@ __global__ __launch_bounds__(256) void adj_density_staged_kernel(AdjArgs a) {
                     slot.rb[e] = a.rhobar[ulo + e];
                 }
                 wave_lds_sync();
-                row_neighbors(
+                const uint32_t e0 = row_first(r.lo[row], r.hi[row]) - ulo;
+                row_neighbors_from(
                     r.lo[row], r.hi[row], r.mk[row],
                     [&](uint32_t j) {
                         const float4 pj = slot.pos[j - ulo];
@@ -401,7 +406,8 @@ __global__ __launch_bounds__(256) void adj_density_staged_kernel(AdjArgs a) {
                     [&](uint32_t j) {
                         const uint32_t e = j - ulo;
                         adj_density_pair(sc, pi, rbi, slot.pos[e], slot.rb[e], gx, gy, gz);
-                    });
+                    },
+                    [&](uint32_t i) { adj_density_pair(sc, pi, rbi, slot.pos[e0 + i], slot.rb[e0 + i], gx, gy, gz); });
                 wave_lds_sync();
             } else {
                 row_neighbors(
@@ -421,8 +427,8 @@ __global__ void adj_density_kernel(AdjArgs a) {
     uint32_t t0, t1;
     tile_span(a.ntiles, t0, t1);
     for (uint32_t t = t0; t < t1; ++t) {
-        const uint2 ti = a.tiles[t];
-        const SceneConst& sc = a.scenes[ti.x];
+        const uint2 ti = tile_at(a.tiles, t);
+        const SceneConst sc = scene_at(a.scenes, ti.x);
         const uint32_t k = ti.y + threadIdx.x;
         if (k >= sc.p_end) continue;
         const float4 pi = a.pos_s[k];

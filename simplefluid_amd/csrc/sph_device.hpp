@@ -23,8 +23,8 @@
 
 namespace sph {
 
-// Per-scene constants.  The single-scene path passes one by value (kernel
-// arguments live in SGPRs); the batched path indexes a table per lane.
+// Per-scene constants, one table entry per scene (a single-scene handle has
+// a table of one).  Kernels read a tile's entry through scene_at.
 struct SceneConst {
     float bmin[3];
     float inv_cs;      // 1 / (h (1 + 2^-10)), the cell size's reciprocal
@@ -84,6 +84,25 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t bid, uint32_t nblocks) {
     return x * q + (x < r ? x : r) + i;
 }
 
+// A tile's (scene, first particle) and its scene's constants.  Tile and
+// scene are the same for the whole block, and neither table is written
+// while a pass runs, so both are read as constant memory: with a uniform
+// index the loads are scalar and the constants sit in SGPRs, instead of
+// per-lane loads that fences between LDS phases make the compiler repeat.
+typedef __attribute__((address_space(4))) const uint32_t ConstWord;
+__device__ __forceinline__ uint2 tile_at(const uint2* tiles, uint32_t t) {
+    ConstWord* p = (ConstWord*)(tiles + t);
+    return make_uint2(p[0], p[1]);
+}
+__device__ __forceinline__ SceneConst scene_at(const SceneConst* scenes, uint32_t s) {
+    ConstWord* p = (ConstWord*)(scenes + s);
+    SceneConst c;
+    uint32_t* d = reinterpret_cast<uint32_t*>(&c);
+#pragma unroll
+    for (uint32_t i = 0; i < sizeof(SceneConst) / 4; ++i) d[i] = p[i];
+    return c;
+}
+
 // Work over tiles.  A plain launch has one block per tile, in XCD-aware
 // order.  A persistent launch (queue != null, gridDim >= 8) gives each XCD an
 // equal contiguous share of the tiles; the XCD's blocks (blockIdx % 8) take
@@ -103,7 +122,7 @@ __device__ __forceinline__ void for_each_tile(uint32_t ntiles, uint32_t* __restr
     for (;;) {
         if (threadIdx.x == 0) s_next = t_beg + atomicAdd(queue + x, 1u);
         __syncthreads();
-        const uint32_t t = s_next;
+        const uint32_t t = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_next);  // the same in every lane
         __syncthreads();
         if (t >= t_end) break;
         body(t);
@@ -203,37 +222,64 @@ __device__ __forceinline__ void row_neighbors(uint32_t lo, uint32_t hi, uint32_t
     for (uint32_t r = __builtin_bitreverse32(m); r; r &= r - 1u) f(jb + (uint32_t)__builtin_ctz(r));
 }
 
+// The first candidate the mask covers: lo, or for a row of more than 32
+// candidates the first of its last 32.
+__device__ __forceinline__ uint32_t row_first(uint32_t lo, uint32_t hi) {
+    const uint32_t span = mask_span(lo, hi);
+    return span > 32u ? lo + span - 32u : lo;
+}
+
+// row_neighbors with the mask's visits given as offsets: g(i) for candidate
+// row_first(lo, hi) + i (f(j) for the re-tested head, as before).  A caller
+// reading the row from a staged copy then adds the row's offset once per
+// row instead of once per neighbour.  Same neighbours, same order.
+template <class Retest, class F, class G>
+__device__ __forceinline__ void row_neighbors_from(uint32_t lo, uint32_t hi, uint32_t m, Retest&& retest, F&& f,
+                                                   G&& g) {
+    const uint32_t span = mask_span(lo, hi);
+    const uint32_t j0 = row_first(lo, hi);
+    for (uint32_t j = lo; j < j0; ++j)
+        if (retest(j)) f(j);
+    // Reversed, bit i is candidate lo + span - 32 + i; a row of fewer than
+    // 32 slots drops the 32 - span empty low bits, so bit i is lo + i.
+    uint32_t r = __builtin_bitreverse32(m);
+    if (span - 1u < 31u) r >>= 32u - span;
+    for (; r; r &= r - 1u) g((uint32_t)__builtin_ctz(r));
+}
+
 
 // Wave-wide minimum / maximum of one value per lane (all 64 lanes active):
-// DPP within each row of 16 lanes, then the four rows' results.  The
-// permutations read no lane outside the row, so the "old" operand (the
-// operation's identity) is never used, and the compiler folds each move
-// into the min / max as its DPP source.
-__device__ __forceinline__ uint32_t row16_min(uint32_t v) {
-    v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)v, 0xB1, 0xF, 0xF, false));   // quad_perm 1,0,3,2
-    v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)v, 0x4E, 0xF, 0xF, false));   // quad_perm 2,3,0,1
-    v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)v, 0x141, 0xF, 0xF, false));  // row_half_mirror
-    v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)v, 0x140, 0xF, 0xF, false));  // row_mirror
-    return v;
+// DPP within each row of 16 lanes, then row broadcasts fold row 0 into 1 and
+// row 2 into 3 (lane 15 and 47 to the odd rows), then rows 0-1 into 2-3
+// (lane 31 to rows 2 and 3), so lane 63 holds the result.  The operation's
+// identity is the "old" operand: rows the row mask leaves out take it and
+// keep their own value, and the compiler folds each move into the min / max
+// as its DPP source.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ uint32_t dpp_min(uint32_t v) {
+    return min(v, (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)v, CTRL, ROWS, 0xF, false));
 }
-__device__ __forceinline__ uint32_t row16_max(uint32_t v) {
-    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false));
-    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false));
-    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x141, 0xF, 0xF, false));
-    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x140, 0xF, 0xF, false));
-    return v;
+template <int CTRL, int ROWS>
+__device__ __forceinline__ uint32_t dpp_max(uint32_t v) {
+    return max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROWS, 0xF, false));
 }
 __device__ __forceinline__ uint32_t wave_umin(uint32_t v) {
-    v = row16_min(v);
-    const uint32_t a0 = (uint32_t)__builtin_amdgcn_readlane((int)v, 0), a1 = (uint32_t)__builtin_amdgcn_readlane((int)v, 16);
-    const uint32_t a2 = (uint32_t)__builtin_amdgcn_readlane((int)v, 32), a3 = (uint32_t)__builtin_amdgcn_readlane((int)v, 48);
-    return min(min(a0, a1), min(a2, a3));
+    v = dpp_min<0xB1, 0xF>(v);   // quad_perm 1,0,3,2
+    v = dpp_min<0x4E, 0xF>(v);   // quad_perm 2,3,0,1
+    v = dpp_min<0x141, 0xF>(v);  // row_half_mirror
+    v = dpp_min<0x140, 0xF>(v);  // row_mirror
+    v = dpp_min<0x142, 0xA>(v);  // row_bcast:15
+    v = dpp_min<0x143, 0xC>(v);  // row_bcast:31
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
 }
 __device__ __forceinline__ uint32_t wave_umax(uint32_t v) {
-    v = row16_max(v);
-    const uint32_t a0 = (uint32_t)__builtin_amdgcn_readlane((int)v, 0), a1 = (uint32_t)__builtin_amdgcn_readlane((int)v, 16);
-    const uint32_t a2 = (uint32_t)__builtin_amdgcn_readlane((int)v, 32), a3 = (uint32_t)__builtin_amdgcn_readlane((int)v, 48);
-    return max(max(a0, a1), max(a2, a3));
+    v = dpp_max<0xB1, 0xF>(v);
+    v = dpp_max<0x4E, 0xF>(v);
+    v = dpp_max<0x141, 0xF>(v);
+    v = dpp_max<0x140, 0xF>(v);
+    v = dpp_max<0x142, 0xA>(v);
+    v = dpp_max<0x143, 0xC>(v);
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
 }
 
 // LDS order between a wave's own stores and its later loads of other

@@ -356,15 +356,17 @@ __device__ __forceinline__ uint32_t force_one_staged(const StepArgs& a, const Sc
             }
             wave_lds_sync();
             // Entry j of the row sits at slot.pos[j - ulo], its velocity
-            // kStage records further on.
+            // kStage records further on; the mask's entries from row_first.
             const float4* base = slot.pos - ulo;
-            row_neighbors(
+            const float4* bm = slot.pos + (row_first(lo[row], hi[row]) - ulo);
+            row_neighbors_from(
                 lo[row], hi[row], mk[row],
                 [&](uint32_t j) {
                     const float4 pj = base[j];
                     return j != k && dist2(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z) < sc.h2;
                 },
-                [&](uint32_t j) { force_pair(sc, pi, vi, base[j], base[j + kStage], f); });
+                [&](uint32_t j) { force_pair(sc, pi, vi, base[j], base[j + kStage], f); },
+                [&](uint32_t i) { force_pair(sc, pi, vi, bm[i], bm[i + kStage], f); });
             wave_lds_sync();
         } else {
             row_neighbors(
@@ -388,8 +390,8 @@ __global__ __launch_bounds__(BLOCK) void density_kernel(StepArgs a) {
     const __amdgpu_buffer_rsrc_t rz = buffer_of(a.soa + 2 * (size_t)a.n, a.n);
     const __amdgpu_buffer_rsrc_t rcs = buffer_of(a.cstart, a.ncells + 1u);
     for_each_tile(a.ntiles, a.queue, [&](uint32_t t) {
-        const uint2 ti = a.tiles[t];
-        const SceneConst& sc = a.scenes[ti.x];
+        const uint2 ti = tile_at(a.tiles, t);
+        const SceneConst sc = scene_at(a.scenes, ti.x);
         const uint32_t k = ti.y + threadIdx.x;
         if (k < sc.p_end) density_one(a, sc, k, rpos, rx, ry, rz, rcs);
     });
@@ -405,8 +407,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8, 8))) v
     const __amdgpu_buffer_rsrc_t rcs = buffer_of(a.cstart, a.ncells + 1u);
     DensitySlot& slot = s_slot[threadIdx.x >> 6];
     for_each_tile(a.ntiles, a.queue, [&](uint32_t t) {
-        const uint2 ti = a.tiles[t];
-        const SceneConst& sc = a.scenes[ti.x];
+        const uint2 ti = tile_at(a.tiles, t);
+        const SceneConst sc = scene_at(a.scenes, ti.x);
         const uint32_t k = ti.y + threadIdx.x;
         density_one_staged(a, sc, k, k < sc.p_end, rpos, rcs, slot);
     });
@@ -423,8 +425,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8, 8))) v
     const __amdgpu_buffer_rsrc_t rcs = buffer_of(a.cstart, a.ncells + 1u);
     uint32_t* queue = a.queue ? a.queue + 8 : nullptr;
     for_each_tile(a.ntiles, queue, [&](uint32_t t) {
-        const uint2 ti = a.tiles[t];
-        const SceneConst& sc = a.scenes[ti.x];
+        const uint2 ti = tile_at(a.tiles, t);
+        const SceneConst sc = scene_at(a.scenes, ti.x);
         const uint32_t k = ti.y + threadIdx.x;
         const bool active = k < sc.p_end;
         uint32_t key = 0;
@@ -454,8 +456,8 @@ __global__ __launch_bounds__(BLOCK) void force_staged_kernel(StepArgs a) {
     uint32_t* queue = a.queue ? a.queue + 8 : nullptr;
     StageSlot& slot = s_slot[threadIdx.x >> 6];
     for_each_tile(a.ntiles, queue, [&](uint32_t t) {
-        const uint2 ti = a.tiles[t];
-        const SceneConst& sc = a.scenes[ti.x];
+        const uint2 ti = tile_at(a.tiles, t);
+        const SceneConst sc = scene_at(a.scenes, ti.x);
         const uint32_t k = ti.y + threadIdx.x;
         const bool active = k < sc.p_end;
         const uint32_t key = force_one_staged(a, sc, k, active, rpos, rvel, rcs, slot);
@@ -479,8 +481,8 @@ __global__ void hash_kernel(const float4* __restrict__ pos, const float4* __rest
     uint32_t t0, t1;
     tile_span(ntiles, t0, t1);
     for (uint32_t t = t0; t < t1; ++t) {
-        const uint2 ti = tiles[t];
-        const SceneConst& sc = scenes[ti.x];
+        const uint2 ti = tile_at(tiles, t);
+        const SceneConst sc = scene_at(scenes, ti.x);
         const uint32_t k = ti.y + threadIdx.x;
         const bool active = k < sc.p_end;
         uint32_t key = 0;
@@ -509,8 +511,8 @@ __global__ void neighbors_kernel(StepArgs a, const uint32_t* __restrict__ offset
     uint32_t t0, t1;
     tile_span(a.ntiles, t0, t1);
     for (uint32_t t = t0; t < t1; ++t) {
-        const uint2 ti = a.tiles[t];
-        const SceneConst& sc = a.scenes[ti.x];
+        const uint2 ti = tile_at(a.tiles, t);
+        const SceneConst sc = scene_at(a.scenes, ti.x);
         const uint32_t k = ti.y + threadIdx.x;
         if (k >= sc.p_end) continue;
         const float4 pi = a.pos_s[k];
