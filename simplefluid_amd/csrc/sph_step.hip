@@ -445,9 +445,10 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8, 8))) v
     });
 }
 
-// force_kernel with the rows staged in LDS (4 KB per wave).
+// force_kernel with the rows staged in LDS (4 KB per wave).  Six waves per
+// SIMD: the bound fits it in 80 VGPRs without spills (82, five waves, without).
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void force_staged_kernel(StepArgs a) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6, 6))) void force_staged_kernel(StepArgs a) {
     if (a.halt && *a.halt) return;
     __shared__ StageSlot s_slot[BLOCK / 64];
     const __amdgpu_buffer_rsrc_t rpos = buffer_of(a.pos_s, a.n);
