@@ -201,14 +201,19 @@ def test_backward_refuses_a_stale_rollout():
     e.close()
 
 
-def test_batched_adjoint_equals_single_scene():
+@pytest.mark.parametrize("staged", [0, 1])
+def test_batched_adjoint_equals_single_scene(staged):
+    """A batch's gradients equal each scene's own handle's bit for bit, with
+    the rows gathered and with them staged in LDS (forward and adjoint)."""
     from simplefluid_amd import engine
     cases = [case(n3=(8, 10, 8), seed=s, B=40.0 + 20 * s, mu=0.01 * (s + 1)) for s in range(3)]
     steps, every = 16, 5
+    tuning = (256, 256, 256, 8, 0, staged, staged)
     b = engine.BatchEngine([c[0] for c in cases], [len(c[1]) for c in cases])
     b.set_cell_subdivision(4)   # same grid as the single-scene handles below
     b.set_particles(np.concatenate([c[1] for c in cases]), np.concatenate([c[2] for c in cases]))
     b.make_ready()
+    b.set_tuning(*tuning)
     b.rollout_forward(steps, every)
     xT = b.get_particles().astype(np.float64)
     tgt = np.concatenate([c[3] for c in cases])
@@ -219,7 +224,7 @@ def test_batched_adjoint_equals_single_scene():
         g[sl] = 2 * (1.0 / (3 * (off[i + 1] - off[i]))) * (xT[sl] - tgt[sl])   # as gpu_grads forms it
     dB, dmu, dv0 = b.rollout_backward(g.astype(np.float32))
     for i, (p, pos, vel, target) in enumerate(cases):
-        _, sB, smu, sdv0, _, _ = gpu_grads(p, pos, vel, target, steps, every, xsub=4)
+        _, sB, smu, sdv0, _, _ = gpu_grads(p, pos, vel, target, steps, every, xsub=4, tuning=tuning)
         assert dB[i] == np.float32(sB) and dmu[i] == np.float32(smu)
         assert np.array_equal(dv0[off[i]:off[i + 1]], sdv0.astype(np.float32))
 
