@@ -282,6 +282,18 @@ __device__ __forceinline__ uint32_t wave_umax(uint32_t v) {
     return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
 }
 
+// Direct copy of 16 bytes at byte offset off of a buffer into LDS: lane i
+// writes dst[i] (the wave's active lanes; dst is the same for all lanes).
+// No registers hold the data; lds_copy_wait() waits for the wave's copies.
+typedef __attribute__((address_space(3))) void lds_void;
+__device__ __forceinline__ void lds_copy16(__amdgpu_buffer_rsrc_t r, float4* dst, uint32_t off) {
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (lds_void*)dst, 16, (int)off, 0, 0, 0);
+}
+__device__ __forceinline__ void lds_copy_wait() {
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0); expcnt and lgkmcnt unconstrained
+    __builtin_amdgcn_wave_barrier();
+}
+
 // LDS order between a wave's own stores and its later loads of other
 // lanes' entries (and back).
 __device__ __forceinline__ void wave_lds_sync() {

@@ -109,6 +109,7 @@ __device__ __forceinline__ void density_one(const StepArgs& a, const SceneConst&
 // floats per plane, a trip's reads of all three planes fit the 8-bit dword
 // offsets of ds_read2_b32 from one address (z at 248..251).
 constexpr uint32_t kStageD = 120;
+static_assert(kStageD <= 128, "the copy into a slot takes two records per lane");
 struct DensitySlot {
     float x[kStageD + 4];
     float y[kStageD + 4];
@@ -151,21 +152,47 @@ __device__ __forceinline__ void density_one_staged(const StepArgs& a, const Scen
     const uint32_t lane = threadIdx.x & 63u;
     const float h2 = sc.h2;
     float sum = 0.0f;
+    // Row r + 1's records are loaded into registers while row r's trips run
+    // from the slot, and stored into the slot once those trips are done, so
+    // each copy's latency hides behind the previous row's work.  A copy holds
+    // at most 2 x 64 records: lanes take entries lane and lane + 64.
+    uint32_t nlo, nhi;
+    bool nfit;
+    float4 n0 = make_float4(0.f, 0.f, 0.f, 0.f), n1 = n0;
+    auto fetch = [&](int row) {
+        const bool has = hi[row] > lo[row];
+        nlo = wave_umin(has ? lo[row] : 0xffffffffu);
+        nhi = wave_umax(has ? hi[row] : 0u);
+        nfit = nlo < nhi && nhi - nlo <= kStageD;
+        if (nfit) {
+            if (lane < nhi - nlo) n0 = load4(rpos, (nlo + lane) << 4);
+            if (lane + 64u < nhi - nlo) n1 = load4(rpos, (nlo + lane + 64u) << 4);
+        }
+    };
+    fetch(0);
 #pragma unroll
     for (int row = 0; row < kRows; ++row) {
         uint32_t m = 0u;
         const uint32_t cnt = hi[row] - lo[row];
         const bool has = cnt > 0u;
-        const uint32_t ulo = wave_umin(has ? lo[row] : 0xffffffffu);
-        const uint32_t uhi = wave_umax(has ? hi[row] : 0u);
-        if (ulo < uhi && uhi - ulo <= kStageD) {
-            for (uint32_t e = lane; e < uhi - ulo; e += 64u) {
-                const float4 p = load4(rpos, (ulo + e) << 4);
-                slot.x[e] = p.x;
-                slot.y[e] = p.y;
-                slot.z[e] = p.z;
+        const uint32_t ulo = nlo, uhi = nhi;
+        const bool fit = nfit;
+        if (fit) {
+            const uint32_t span = uhi - ulo;
+            if (lane < span) {
+                slot.x[lane] = n0.x;
+                slot.y[lane] = n0.y;
+                slot.z[lane] = n0.z;
+            }
+            if (lane + 64u < span) {
+                slot.x[lane + 64u] = n1.x;
+                slot.y[lane + 64u] = n1.y;
+                slot.z[lane + 64u] = n1.z;
             }
             wave_lds_sync();
+        }
+        if (row + 1 < kRows) fetch(row + 1);
+        if (fit) {
             uint32_t o = lo[row] - ulo;
             const uint32_t full_end = o + (cnt & ~3u);
             for (; o != full_end; o += 4u)
@@ -308,9 +335,12 @@ __device__ __forceinline__ uint32_t force_finish(const StepArgs& a, const SceneC
 
 // Candidates a wave stages per row: its lanes' windows of one row lie in
 // one sorted range of about 64 + 2 chords of particles when the wave sits in
-// one y/z column; a wider range (a wave across two columns) is walked from
-// global memory as in force_one.
-constexpr uint32_t kStage = 128;
+// one y/z column (73 on average on the 1M dam break, 99 at the 99th
+// percentile of the 4.4M sphere drop); a wider range (a wave across two
+// columns) is walked from global memory as in force_one.  A wave has two
+// slots, one filled while the other is walked.
+constexpr uint32_t kStage = 96;
+static_assert(kStage <= 128, "the copy into a slot takes two records per lane");
 struct StageSlot {
     float4 pos[kStage];
     float4 vel[kStage];
@@ -318,14 +348,14 @@ struct StageSlot {
 
 // force_one with each row's candidates read from LDS: per row, the wave
 // copies the union of its lanes' windows (contiguous, so the copy is
-// coalesced) into its slot, then every lane walks its own mask over the
+// coalesced) into a slot, then every lane walks its own mask over the
 // slot.  Same pairs, same order, same arithmetic: the results are
 // force_one's bit for bit.  Every lane of the wave calls it (the copy and
 // the wave-wide bounds need them all); inactive lanes have empty windows.
 __device__ __forceinline__ uint32_t force_one_staged(const StepArgs& a, const SceneConst& sc, uint32_t k,
                                                      bool active, __amdgpu_buffer_rsrc_t rpos,
                                                      __amdgpu_buffer_rsrc_t rvel, __amdgpu_buffer_rsrc_t rcs,
-                                                     StageSlot& slot) {
+                                                     StageSlot& slot_a, StageSlot& slot_b) {
     // Loads past the buffer end return zeros.
     const float4 pi = load4(rpos, k << 4);
     const float4 vi = load4(rvel, k << 4);
@@ -343,18 +373,39 @@ __device__ __forceinline__ uint32_t force_one_staged(const StepArgs& a, const Sc
     }
     const uint32_t lane = threadIdx.x & 63u;
     ForceAcc f = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    // Each row's union of windows is copied into one of two slots by direct
+    // loads into LDS (no registers), row r + 1's copy issued before row r is
+    // walked, so the copy runs behind the walk.  Lane i of a copy writes
+    // entry i (and i + 64) of the slot.
+    uint32_t nlo, nhi;
+    bool nfit;
+    auto fetch = [&](int row) {
+        const bool has = hi[row] > lo[row];
+        nlo = wave_umin(has ? lo[row] : 0xffffffffu);
+        nhi = wave_umax(has ? hi[row] : 0u);
+        nfit = nlo < nhi && nhi - nlo <= kStage;
+        if (!nfit) return;
+        StageSlot& sl = (row & 1) ? slot_b : slot_a;
+        const uint32_t span = nhi - nlo;
+        if (lane < span) {
+            lds_copy16(rpos, sl.pos, (nlo + lane) << 4);
+            lds_copy16(rvel, sl.vel, (nlo + lane) << 4);
+        }
+        if (span > 64u && lane + 64u < span) {
+            lds_copy16(rpos, sl.pos + 64, (nlo + lane + 64u) << 4);
+            lds_copy16(rvel, sl.vel + 64, (nlo + lane + 64u) << 4);
+        }
+    };
+    fetch(0);
 #pragma unroll
     for (int row = 0; row < kRows; ++row) {
-        const bool has = hi[row] > lo[row];
-        const uint32_t ulo = wave_umin(has ? lo[row] : 0xffffffffu);
-        const uint32_t uhi = wave_umax(has ? hi[row] : 0u);
+        const uint32_t ulo = nlo, uhi = nhi;
+        const bool fit = nfit;
+        StageSlot& slot = (row & 1) ? slot_b : slot_a;
+        if (fit) lds_copy_wait();
+        if (row + 1 < kRows) fetch(row + 1);
         if (ulo >= uhi) continue;
-        if (uhi - ulo <= kStage) {
-            for (uint32_t e = lane; e < uhi - ulo; e += 64u) {
-                slot.pos[e] = load4(rpos, (ulo + e) << 4);
-                slot.vel[e] = load4(rvel, (ulo + e) << 4);
-            }
-            wave_lds_sync();
+        if (fit) {
             // Entry j of the row sits at slot.pos[j - ulo], its velocity
             // kStage records further on; the mask's entries from row_first.
             const float4* base = slot.pos - ulo;
@@ -367,7 +418,6 @@ __device__ __forceinline__ uint32_t force_one_staged(const StepArgs& a, const Sc
                 },
                 [&](uint32_t j) { force_pair(sc, pi, vi, base[j], base[j + kStage], f); },
                 [&](uint32_t i) { force_pair(sc, pi, vi, bm[i], bm[i + kStage], f); });
-            wave_lds_sync();
         } else {
             row_neighbors(
                 lo[row], hi[row], mk[row],
@@ -445,23 +495,26 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8, 8))) v
     });
 }
 
-// force_kernel with the rows staged in LDS (4 KB per wave).  Six waves per
-// SIMD: the bound fits it in 80 VGPRs without spills (82, five waves, without).
+// force_kernel with the rows staged in LDS (two 3 KB slots per wave).  Six
+// waves per SIMD: the bound fits it in 80 VGPRs (one spilled; 82 and five
+// waves without it).
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6, 6))) void force_staged_kernel(StepArgs a) {
     if (a.halt && *a.halt) return;
-    __shared__ StageSlot s_slot[BLOCK / 64];
+    __shared__ StageSlot s_slot_a[BLOCK / 64];
+    __shared__ StageSlot s_slot_b[BLOCK / 64];
     const __amdgpu_buffer_rsrc_t rpos = buffer_of(a.pos_s, a.n);
     const __amdgpu_buffer_rsrc_t rvel = buffer_of(a.vel_s, a.n);
     const __amdgpu_buffer_rsrc_t rcs = buffer_of(a.cstart, a.ncells + 1u);
     uint32_t* queue = a.queue ? a.queue + 8 : nullptr;
-    StageSlot& slot = s_slot[threadIdx.x >> 6];
+    StageSlot& slot_a = s_slot_a[threadIdx.x >> 6];
+    StageSlot& slot_b = s_slot_b[threadIdx.x >> 6];
     for_each_tile(a.ntiles, queue, [&](uint32_t t) {
         const uint2 ti = tile_at(a.tiles, t);
         const SceneConst sc = scene_at(a.scenes, ti.x);
         const uint32_t k = ti.y + threadIdx.x;
         const bool active = k < sc.p_end;
-        const uint32_t key = force_one_staged(a, sc, k, active, rpos, rvel, rcs, slot);
+        const uint32_t key = force_one_staged(a, sc, k, active, rpos, rvel, rcs, slot_a, slot_b);
         const uint32_t cslot = count_slot(a.count, a.count + a.ncells, key, active);
         if (a.vmax2) {
             float4 vn = make_float4(0.f, 0.f, 0.f, 0.f);
