@@ -190,6 +190,7 @@ __device__ __forceinline__ double2 adj_force_one(const AdjArgs& a, const SceneCo
 // arithmetic: the gathered kernels' results bit for bit.  Every lane of the
 // wave takes part; inactive lanes have empty windows and store nothing.
 constexpr uint32_t kStageA = 96;
+static_assert(kStageA <= 128, "the copy into a slot takes two records per lane");
 struct AdjForceSlot {
     float4 pos[kStageA];
     float4 vel[kStageA];
@@ -234,10 +235,30 @@ __device__ __forceinline__ double2 adj_force_one_staged(const AdjArgs& a, const 
         const uint32_t uhi = wave_umax(has ? r.hi[row] : 0u);
         if (ulo >= uhi) continue;
         if (uhi - ulo <= kStageA) {
-            for (uint32_t e = lane; e < uhi - ulo; e += 64u) {
-                slot.pos[e] = a.pos_s[ulo + e];
-                slot.vel[e] = a.vel_s[ulo + e];
-                slot.abar[e] = a.abar[ulo + e];
+            // Both halves of the copy (a union holds at most 2 x 64 records)
+            // are loaded before either is stored, so their latencies overlap.
+            const uint32_t span = uhi - ulo;
+            const bool c0 = lane < span, c1 = lane + 64u < span;
+            float4 p0 = make_float4(0.f, 0.f, 0.f, 0.f), v0 = p0, b0 = p0, p1 = p0, v1 = p0, b1 = p0;
+            if (c0) {
+                p0 = a.pos_s[ulo + lane];
+                v0 = a.vel_s[ulo + lane];
+                b0 = a.abar[ulo + lane];
+            }
+            if (c1) {
+                p1 = a.pos_s[ulo + lane + 64u];
+                v1 = a.vel_s[ulo + lane + 64u];
+                b1 = a.abar[ulo + lane + 64u];
+            }
+            if (c0) {
+                slot.pos[lane] = p0;
+                slot.vel[lane] = v0;
+                slot.abar[lane] = b0;
+            }
+            if (c1) {
+                slot.pos[lane + 64u] = p1;
+                slot.vel[lane + 64u] = v1;
+                slot.abar[lane + 64u] = b1;
             }
             wave_lds_sync();
             const uint32_t e0 = row_first(r.lo[row], r.hi[row]) - ulo;
