@@ -38,8 +38,11 @@ sys.path.insert(0, ROOT)
 
 METRIC = "particle-steps/s @1M (plain vs batched); % HBM roofline; adjoint/fwd ratio"
 HBM_PEAK_GBS = 8000.0
-# Non-packed FP32 VALU issue: 256 CUs x 4 SIMD x 16 lanes x 2.4 GHz.
-VALU_PEAK_TOPS = 39.32
+# MI355X's published FP32 vector rate, 157.3 TFLOP/s, as lane operations (a
+# fused multiply-add is one): 78.6 T lane-op/s.  These passes issue a wave64
+# VALU instruction every 3.3-3.7 SIMD cycles on average, so their mix runs
+# between the 4-cycle rate of a 16-lane SIMD (39.3 T) and this figure.
+VALU_PEAK_TOPS = 78.64
 # Algorithmic bytes per particle and pass (FP32, unpadded float3): hash 20,
 # sort 16 (one ideal pass), reorder 52, cell table 4 + 8 per occupied cell,
 # density 16, force 40, integrate + clamp 60.  The engine fuses hash into the
