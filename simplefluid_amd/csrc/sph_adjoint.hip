@@ -388,7 +388,8 @@ __device__ __forceinline__ void adj_density_finish(const AdjArgs& a, const Scene
 }
 
 // adj_density_kernel with the rows staged in LDS (1.9 KB per wave).
-__global__ __launch_bounds__(256) void adj_density_staged_kernel(AdjArgs a) {
+// Eight waves per SIMD: the bound fits it in 64 VGPRs.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) void adj_density_staged_kernel(AdjArgs a) {
     uint32_t t0, t1;
     tile_span(a.ntiles, t0, t1);
     __shared__ AdjDensitySlot s_slot[256 / 64];
@@ -412,9 +413,26 @@ __global__ __launch_bounds__(256) void adj_density_staged_kernel(AdjArgs a) {
             const uint32_t uhi = wave_umax(has ? r.hi[row] : 0u);
             if (ulo >= uhi) continue;
             if (uhi - ulo <= kStageA) {
-                for (uint32_t e = lane; e < uhi - ulo; e += 64u) {
-                    slot.pos[e] = a.pos_s[ulo + e];
-                    slot.rb[e] = a.rhobar[ulo + e];
+                // Both halves of the copy are loaded before either is stored.
+                const uint32_t span = uhi - ulo;
+                const bool c0 = lane < span, c1 = lane + 64u < span;
+                float4 p0 = make_float4(0.f, 0.f, 0.f, 0.f), p1 = p0;
+                float q0 = 0.f, q1 = 0.f;
+                if (c0) {
+                    p0 = a.pos_s[ulo + lane];
+                    q0 = a.rhobar[ulo + lane];
+                }
+                if (c1) {
+                    p1 = a.pos_s[ulo + lane + 64u];
+                    q1 = a.rhobar[ulo + lane + 64u];
+                }
+                if (c0) {
+                    slot.pos[lane] = p0;
+                    slot.rb[lane] = q0;
+                }
+                if (c1) {
+                    slot.pos[lane + 64u] = p1;
+                    slot.rb[lane + 64u] = q1;
                 }
                 wave_lds_sync();
                 const uint32_t e0 = row_first(r.lo[row], r.hi[row]) - ulo;
