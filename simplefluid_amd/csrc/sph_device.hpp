@@ -247,7 +247,6 @@ __device__ __forceinline__ void row_neighbors_from(uint32_t lo, uint32_t hi, uin
     for (; r; r &= r - 1u) g((uint32_t)__builtin_ctz(r));
 }
 
-
 // Wave-wide minimum / maximum of one value per lane (all 64 lanes active):
 // DPP within each row of 16 lanes, then row broadcasts fold row 0 into 1 and
 // row 2 into 3 (lane 15 and 47 to the odd rows), then rows 0-1 into 2-3
