@@ -78,9 +78,19 @@ __device__ __forceinline__ void density_one(const StepArgs& a, const SceneConst&
         uint32_t off = lo[row] << 2;
         const uint32_t full_end = off + ((cnt & ~3u) << 2);
         // Slots past the row end shift in zero bits (past the array end the
-        // buffer returns 0).
-        for (; off != full_end; off += 16u)
-            density_trip<true>(pi, h2, load4(rx, off), load4(ry, off), load4(rz, off), 4, sum, m);
+        // buffer returns 0).  Each trip's planes are loaded while the trip
+        // before is tested, so a lane has two trips' loads in flight.
+        if (off != full_end) {
+            float4 X = load4(rx, off), Y = load4(ry, off), Z = load4(rz, off);
+            for (off += 16u; off != full_end; off += 16u) {
+                const float4 Xn = load4(rx, off), Yn = load4(ry, off), Zn = load4(rz, off);
+                density_trip<true>(pi, h2, X, Y, Z, 4, sum, m);
+                X = Xn;
+                Y = Yn;
+                Z = Zn;
+            }
+            density_trip<true>(pi, h2, X, Y, Z, 4, sum, m);
+        }
         const int rem = (int)(cnt & 3u);
         if (rem > 0) density_trip<false>(pi, h2, load4(rx, off), load4(ry, off), load4(rz, off), rem, sum, m);
         a.mask[(size_t)row * a.n + k] = m;
@@ -287,15 +297,30 @@ __device__ __forceinline__ uint32_t force_one(const StepArgs& a, const SceneCons
     ForceAcc f = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int row = 0; row < kRows; ++row) {
-        row_neighbors(
-            lo[row], hi[row], mk[row],
-            [&](uint32_t j) {
-                const float4 pj = load4(rpos, j << 4);
-                return j != k && dist2(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z) < sc.h2;
-            },
-            [&](uint32_t j) {
-                force_pair(sc, pi, vi, load4(rpos, j << 4), load4(rvel, j << 4), f);
-            });
+        const uint32_t span = mask_span(lo[row], hi[row]);
+        const uint32_t j0 = row_first(lo[row], hi[row]);
+        for (uint32_t j = lo[row]; j < j0; ++j) {
+            const float4 pj = load4(rpos, j << 4);
+            if (j != k && dist2(pi.x - pj.x, pi.y - pj.y, pi.z - pj.z) < sc.h2)
+                force_pair(sc, pi, vi, pj, load4(rvel, j << 4), f);
+        }
+        // The mask's neighbours in ascending order (bit i: candidate j0 + i),
+        // each one's records loaded while the one before is summed, so a
+        // lane has two neighbours' loads in flight.
+        uint32_t r = __builtin_bitreverse32(mk[row]);
+        if (span - 1u < 31u) r >>= 32u - span;
+        if (r) {
+            uint32_t j = j0 + (uint32_t)__builtin_ctz(r);
+            float4 pj = load4(rpos, j << 4), vj = load4(rvel, j << 4);
+            for (r &= r - 1u; r; r &= r - 1u) {
+                j = j0 + (uint32_t)__builtin_ctz(r);
+                const float4 pn = load4(rpos, j << 4), vn = load4(rvel, j << 4);
+                force_pair(sc, pi, vi, pj, vj, f);
+                pj = pn;
+                vj = vn;
+            }
+            force_pair(sc, pi, vi, pj, vj, f);
+        }
     }
     return force_finish(a, sc, k, pi, vi, f);
 }
