@@ -27,12 +27,9 @@ __device__ __forceinline__ void tile_span(uint32_t ntiles, uint32_t& t0, uint32_
     t1 = t0 + per < ntiles ? t0 + per : ntiles;
 }
 
-// m = (m << 1) | (q < 0), as a compare into VCC and one add with carry
-// (m + m + carry): two instructions where the compiler's select, shift and
-// or take three.  q < 0 is false for a NaN, as d^2 < h^2 is.
-__device__ __forceinline__ uint32_t push_bit(uint32_t m, float q) {
-    asm("v_cmp_gt_f32 vcc, 0, %1\n\tv_addc_co_u32 %0, vcc, %0, %0, vcc" : "+v"(m) : "v"(q) : "vcc");
-    return m;
+// m = (m << 1) | (sign bit of c): one funnel shift (v_alignbit_b32).
+__device__ __forceinline__ uint32_t push_sign(uint32_t m, float c) {
+    return __builtin_amdgcn_alignbit(m, __float_as_uint(c), 31u);
 }
 
 // Candidates come from the x, y, z planes: one 16-byte load per plane gives
@@ -48,15 +45,17 @@ __device__ __forceinline__ void density_trip(const float4& pi, float h2, const f
     for (int u = 0; u < 4; ++u) {
         // q = d2 - h2 is negative exactly when d2 < h2 (an exact tie rounds
         // to +0), and -q = h2 - d2 (rounding is symmetric) is the kernel's t;
-        // slots past the row end get q = 1.  The mask bit is the comparison
-        // q < 0 itself, not q's sign bit: a NaN distance (a candidate with a
+        // slots past the row end get q = 1.  c = min(q, 0) is q where q < 0
+        // and +0 elsewhere, a NaN q included (the minimum of a quiet NaN and
+        // a number is the number): a NaN distance (a candidate with a
         // non-finite coordinate) may carry a set sign bit, and it is no
-        // neighbour, as d2 < h2 says.  t = max(NaN, 0) = 0 adds nothing.
+        // neighbour, as d2 < h2 says.  So c's sign bit is d2 < h2, and -c is
+        // t = max(h2 - d2, 0): c * c * -c adds t^3, and -0 where t is 0.
         float q = dist2(pi.x - xs[u], pi.y - ys[u], pi.z - zs[u]) - h2;
         if (!FULL && u != 0 && u >= rem) q = 1.0f;
-        const float t = fmaxf(-q, 0.0f);
-        sum = fmaf(t * t, t, sum);
-        m = push_bit(m, q);
+        const float c = fminf(q, 0.0f);
+        sum = fmaf(c * c, -c, sum);
+        m = push_sign(m, c);
     }
 }
 
@@ -120,11 +119,17 @@ __device__ __forceinline__ void density_one(const StepArgs& a, const SceneConst&
 // offsets of ds_read2_b32 from one address (z at 248..251).
 constexpr uint32_t kStageD = 120;
 static_assert(kStageD <= 128, "the copy into a slot takes two records per lane");
+constexpr uint32_t kPlaneD = kStageD + 4;
 struct DensitySlot {
-    float x[kStageD + 4];
-    float y[kStageD + 4];
-    float z[kStageD + 4];
+    float v[3 * kPlaneD];  // the x plane, then y, then z
 };
+
+// Four consecutive slot entries of plane d (0 x, 1 y, 2 z) from the x
+// plane's address of the first.
+__device__ __forceinline__ float4 slot_quad(const float* p, int d) {
+    const float* q = p + d * kPlaneD;
+    return make_float4(q[0], q[1], q[2], q[3]);
+}
 
 // Four consecutive sorted positions as x, y, z quadruples (the gathered
 // trip's planes, read from the position records instead).
@@ -190,31 +195,28 @@ __device__ __forceinline__ void density_one_staged(const StepArgs& a, const Scen
         if (fit) {
             const uint32_t span = uhi - ulo;
             if (lane < span) {
-                slot.x[lane] = n0.x;
-                slot.y[lane] = n0.y;
-                slot.z[lane] = n0.z;
+                slot.v[lane] = n0.x;
+                slot.v[kPlaneD + lane] = n0.y;
+                slot.v[2 * kPlaneD + lane] = n0.z;
             }
             if (lane + 64u < span) {
-                slot.x[lane + 64u] = n1.x;
-                slot.y[lane + 64u] = n1.y;
-                slot.z[lane + 64u] = n1.z;
+                slot.v[lane + 64u] = n1.x;
+                slot.v[kPlaneD + lane + 64u] = n1.y;
+                slot.v[2 * kPlaneD + lane + 64u] = n1.z;
             }
             wave_lds_sync();
         }
         if (row + 1 < kRows) fetch(row + 1);
         if (fit) {
-            uint32_t o = lo[row] - ulo;
-            const uint32_t full_end = o + (cnt & ~3u);
-            for (; o != full_end; o += 4u)
-                density_trip<true>(pi, h2, make_float4(slot.x[o], slot.x[o + 1], slot.x[o + 2], slot.x[o + 3]),
-                                   make_float4(slot.y[o], slot.y[o + 1], slot.y[o + 2], slot.y[o + 3]),
-                                   make_float4(slot.z[o], slot.z[o + 1], slot.z[o + 2], slot.z[o + 3]), 4, sum, m);
+            // One induction variable, the trip's slot address, is also the
+            // loop's exit test.
+            const float* p = slot.v + (lo[row] - ulo);
+            const float* const full_end = p + (cnt & ~3u);
+            for (; p != full_end; p += 4)
+                density_trip<true>(pi, h2, slot_quad(p, 0), slot_quad(p, 1), slot_quad(p, 2), 4, sum, m);
             const int rem = (int)(cnt & 3u);
             if (rem > 0)
-                density_trip<false>(pi, h2, make_float4(slot.x[o], slot.x[o + 1], slot.x[o + 2], slot.x[o + 3]),
-                                    make_float4(slot.y[o], slot.y[o + 1], slot.y[o + 2], slot.y[o + 3]),
-                                    make_float4(slot.z[o], slot.z[o + 1], slot.z[o + 2], slot.z[o + 3]), rem, sum,
-                                    m);
+                density_trip<false>(pi, h2, slot_quad(p, 0), slot_quad(p, 1), slot_quad(p, 2), rem, sum, m);
             wave_lds_sync();
         } else if (has) {
             // Past the buffer end the records read as zeros, as the planes do.
