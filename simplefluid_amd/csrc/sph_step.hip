@@ -491,11 +491,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8, 8))) v
     });
 }
 
-// Eight waves per SIMD (64 VGPRs, a 32-byte spill): measured 4.5 % faster
-// than the same kernel without the bound (86.8 against 90.9 us on the 1M
-// double dam break).
+// Six waves per SIMD (77 VGPRs, no VGPR spills).
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8, 8))) void force_kernel(StepArgs a) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6, 6))) void force_kernel(StepArgs a) {
     if (a.halt && *a.halt) return;
     const __amdgpu_buffer_rsrc_t rpos = buffer_of(a.pos_s, a.n);
     const __amdgpu_buffer_rsrc_t rvel = buffer_of(a.vel_s, a.n);
