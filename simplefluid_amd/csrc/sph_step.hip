@@ -167,18 +167,53 @@ __device__ __forceinline__ void density_one_staged(const StepArgs& a, const Scen
     const uint32_t lane = threadIdx.x & 63u;
     const float h2 = sc.h2;
     float sum = 0.0f;
+    // A wave whose active lanes (a prefix of the wave) share one y/z column
+    // holds their x cells in ascending order, so every lane's window of a row
+    // lies in the row's x cells [cx0 - wmax, cx1 + 1 + wmax]: cx0 and cx1 the
+    // first and last lanes' cells, wmax above every lane's half chord in x
+    // cells.  Such a wave bounds each row's union by two scalar reads of the
+    // cell-start table (a superset of the exact union, copied whole) instead
+    // of a wave-wide minimum and maximum, which the other waves, and rows
+    // whose superset does not fit the slot, still take.
+    const uint64_t act = __ballot(active);
+    const uint32_t row0 = __builtin_amdgcn_readfirstlane(g.row0);
+    const bool one_col = act != 0u && __ballot(active && g.row0 != row0) == 0u;
+    uint32_t x0c = 0u, x1c = 0u;
+    if (one_col) {
+        const float gmax = (float)(sc.G[0] - 1);
+        const int cx = (int)fminf(fmaxf(g.fx, 0.0f), gmax);
+        const int cx0 = __builtin_amdgcn_readfirstlane(cx);
+        const int cx1 = __builtin_amdgcn_readlane(cx, 63 - __builtin_clzll(act));
+        // sqrt(hc2) bounds every lane's sqrt(w2) up to rounding, which the
+        // added 1/16 cell covers.
+        const float wmax = __builtin_amdgcn_sqrtf(sc.hc2) * sc.xsubf + sc.wmargin + 0.0625f;
+        x0c = (uint32_t)(int)fminf(fmaxf((float)cx0 - wmax, 0.0f), gmax);
+        x1c = (uint32_t)(int)fminf(fmaxf((float)cx1 + 1.0f + wmax, 0.0f), gmax);
+    }
     // Row r + 1's records are loaded into registers while row r's trips run
     // from the slot, and stored into the slot once those trips are done, so
     // each copy's latency hides behind the previous row's work.  A copy holds
     // at most 2 x 64 records: lanes take entries lane and lane + 64.
-    uint32_t nlo, nhi;
+    uint32_t nlo = 0u, nhi = 0u;
     bool nfit;
     float4 n0 = make_float4(0.f, 0.f, 0.f, 0.f), n1 = n0;
     auto fetch = [&](int row) {
         const bool has = hi[row] > lo[row];
-        nlo = wave_umin(has ? lo[row] : 0xffffffffu);
-        nhi = wave_umax(has ? hi[row] : 0u);
-        nfit = nlo < nhi && nhi - nlo <= kStageD;
+        nfit = false;
+        if (__ballot(has) == 0u) return;
+        if (one_col) {
+            // Some lane's window of this row is not empty, so the row's
+            // column lies inside the grid and both reads inside the table.
+            const uint32_t c = row0 + (uint32_t)((row / 3 - 1) * sc.G[1] + (row % 3 - 1)) * (uint32_t)sc.G[0];
+            nlo = *(ConstWord*)(a.cstart + (c + x0c));
+            nhi = *(ConstWord*)(a.cstart + (c + x1c + 1u));
+            nfit = nlo < nhi && nhi - nlo <= kStageD;
+        }
+        if (!nfit) {
+            nlo = wave_umin(has ? lo[row] : 0xffffffffu);
+            nhi = wave_umax(has ? hi[row] : 0u);
+            nfit = nlo < nhi && nhi - nlo <= kStageD;
+        }
         if (nfit) {
             if (lane < nhi - nlo) n0 = load4(rpos, (nlo + lane) << 4);
             if (lane + 64u < nhi - nlo) n1 = load4(rpos, (nlo + lane + 64u) << 4);
