@@ -145,6 +145,62 @@ __device__ __forceinline__ void load_quad(__amdgpu_buffer_rsrc_t rpos, uint32_t 
     Z = make_float4(__uint_as_float(p0.z), __uint_as_float(p1.z), __uint_as_float(p2.z), __uint_as_float(p3.z));
 }
 
+// The sorted range the staged density pass copies for one row: the union of
+// the wave's windows of that row.  A wave whose active lanes (a prefix of the
+// wave) share one y/z column holds their x cells in ascending order, so every
+// lane's window of a row lies in the row's x cells [cx0 - wmax, cx1 + 1 +
+// wmax]: cx0 and cx1 the first and last lanes' cells, wmax above every
+// lane's half chord in x cells.  Such a wave bounds each row's union by two
+// scalar reads of the cell-start table (a superset of the exact union, copied
+// whole) instead of a wave-wide minimum and maximum, which the other waves,
+// and rows whose superset does not fit the slot, still take.
+struct WaveColumn {
+    bool one_col;
+    uint32_t row0, x0c, x1c;
+};
+
+__device__ __forceinline__ WaveColumn wave_column(const SceneConst& sc, const RowGeom& g, bool active) {
+    WaveColumn wc;
+    const uint64_t act = __ballot(active);
+    wc.row0 = __builtin_amdgcn_readfirstlane(g.row0);
+    wc.one_col = act != 0u && __ballot(active && g.row0 != wc.row0) == 0u;
+    wc.x0c = wc.x1c = 0u;
+    if (wc.one_col) {
+        const float gmax = (float)(sc.G[0] - 1);
+        const int cx = (int)fminf(fmaxf(g.fx, 0.0f), gmax);
+        const int cx0 = __builtin_amdgcn_readfirstlane(cx);
+        const int cx1 = __builtin_amdgcn_readlane(cx, 63 - __builtin_clzll(act));
+        // sqrt(hc2) bounds every lane's sqrt(w2) up to rounding, which the
+        // added 1/16 cell covers.
+        const float wmax = __builtin_amdgcn_sqrtf(sc.hc2) * sc.xsubf + sc.wmargin + 0.0625f;
+        wc.x0c = (uint32_t)(int)fminf(fmaxf((float)cx0 - wmax, 0.0f), gmax);
+        wc.x1c = (uint32_t)(int)fminf(fmaxf((float)cx1 + 1.0f + wmax, 0.0f), gmax);
+    }
+    return wc;
+}
+
+// [nlo, nhi) of row `row` for a slot of CAP entries (empty when no lane has
+// a window in the row); false when it is empty or does not fit (the row is
+// then read from global memory).  Every lane of the wave calls it.
+template <uint32_t CAP>
+__device__ __forceinline__ bool row_union(const StepArgs& a, const SceneConst& sc, const WaveColumn& wc, int row,
+                                          uint32_t lo, uint32_t hi, uint32_t& nlo, uint32_t& nhi) {
+    const bool has = hi > lo;
+    nlo = nhi = 0u;
+    if (__ballot(has) == 0u) return false;
+    if (wc.one_col) {
+        // Some lane's window of this row is not empty, so the row's column
+        // lies inside the grid and both reads inside the table.
+        const uint32_t c = wc.row0 + (uint32_t)((row / 3 - 1) * sc.G[1] + (row % 3 - 1)) * (uint32_t)sc.G[0];
+        nlo = *(ConstWord*)(a.cstart + (c + wc.x0c));
+        nhi = *(ConstWord*)(a.cstart + (c + wc.x1c + 1u));
+        if (nlo < nhi && nhi - nlo <= CAP) return true;
+    }
+    nlo = wave_umin(has ? lo : 0xffffffffu);
+    nhi = wave_umax(has ? hi : 0u);
+    return nlo < nhi && nhi - nlo <= CAP;
+}
+
 // density_one with each row's candidates read from LDS: per row the wave
 // copies the positions of the union of its lanes' windows into its slot as
 // x, y, z planes (coalesced record loads), then every lane runs its own
@@ -167,29 +223,7 @@ __device__ __forceinline__ void density_one_staged(const StepArgs& a, const Scen
     const uint32_t lane = threadIdx.x & 63u;
     const float h2 = sc.h2;
     float sum = 0.0f;
-    // A wave whose active lanes (a prefix of the wave) share one y/z column
-    // holds their x cells in ascending order, so every lane's window of a row
-    // lies in the row's x cells [cx0 - wmax, cx1 + 1 + wmax]: cx0 and cx1 the
-    // first and last lanes' cells, wmax above every lane's half chord in x
-    // cells.  Such a wave bounds each row's union by two scalar reads of the
-    // cell-start table (a superset of the exact union, copied whole) instead
-    // of a wave-wide minimum and maximum, which the other waves, and rows
-    // whose superset does not fit the slot, still take.
-    const uint64_t act = __ballot(active);
-    const uint32_t row0 = __builtin_amdgcn_readfirstlane(g.row0);
-    const bool one_col = act != 0u && __ballot(active && g.row0 != row0) == 0u;
-    uint32_t x0c = 0u, x1c = 0u;
-    if (one_col) {
-        const float gmax = (float)(sc.G[0] - 1);
-        const int cx = (int)fminf(fmaxf(g.fx, 0.0f), gmax);
-        const int cx0 = __builtin_amdgcn_readfirstlane(cx);
-        const int cx1 = __builtin_amdgcn_readlane(cx, 63 - __builtin_clzll(act));
-        // sqrt(hc2) bounds every lane's sqrt(w2) up to rounding, which the
-        // added 1/16 cell covers.
-        const float wmax = __builtin_amdgcn_sqrtf(sc.hc2) * sc.xsubf + sc.wmargin + 0.0625f;
-        x0c = (uint32_t)(int)fminf(fmaxf((float)cx0 - wmax, 0.0f), gmax);
-        x1c = (uint32_t)(int)fminf(fmaxf((float)cx1 + 1.0f + wmax, 0.0f), gmax);
-    }
+    const WaveColumn wc = wave_column(sc, g, active);
     // Row r + 1's records are loaded into registers while row r's trips run
     // from the slot, and stored into the slot once those trips are done, so
     // each copy's latency hides behind the previous row's work.  A copy holds
@@ -198,22 +232,7 @@ __device__ __forceinline__ void density_one_staged(const StepArgs& a, const Scen
     bool nfit;
     float4 n0 = make_float4(0.f, 0.f, 0.f, 0.f), n1 = n0;
     auto fetch = [&](int row) {
-        const bool has = hi[row] > lo[row];
-        nfit = false;
-        if (__ballot(has) == 0u) return;
-        if (one_col) {
-            // Some lane's window of this row is not empty, so the row's
-            // column lies inside the grid and both reads inside the table.
-            const uint32_t c = row0 + (uint32_t)((row / 3 - 1) * sc.G[1] + (row % 3 - 1)) * (uint32_t)sc.G[0];
-            nlo = *(ConstWord*)(a.cstart + (c + x0c));
-            nhi = *(ConstWord*)(a.cstart + (c + x1c + 1u));
-            nfit = nlo < nhi && nhi - nlo <= kStageD;
-        }
-        if (!nfit) {
-            nlo = wave_umin(has ? lo[row] : 0xffffffffu);
-            nhi = wave_umax(has ? hi[row] : 0u);
-            nfit = nlo < nhi && nhi - nlo <= kStageD;
-        }
+        nfit = row_union<kStageD>(a, sc, wc, row, lo[row], hi[row], nlo, nhi);
         if (nfit) {
             if (lane < nhi - nlo) n0 = load4(rpos, (nlo + lane) << 4);
             if (lane + 64u < nhi - nlo) n1 = load4(rpos, (nlo + lane + 64u) << 4);
