@@ -362,12 +362,14 @@ __device__ __forceinline__ uint32_t force_one(const StepArgs& a, const SceneCons
         }
         // The mask's neighbours in ascending order (bit i: candidate j0 + i),
         // each one's records loaded while the one before is summed, so a
-        // lane has two neighbours' loads in flight.
+        // lane has two neighbours' loads in flight; unrolled twice, the two
+        // register sets swap roles instead of being copied.
         uint32_t r = __builtin_bitreverse32(mk[row]);
         if (span - 1u < 31u) r >>= 32u - span;
         if (r) {
             uint32_t j = j0 + (uint32_t)__builtin_ctz(r);
             float4 pj = load4(rpos, j << 4), vj = load4(rvel, j << 4);
+#pragma unroll 2
             for (r &= r - 1u; r; r &= r - 1u) {
                 j = j0 + (uint32_t)__builtin_ctz(r);
                 const float4 pn = load4(rpos, j << 4), vn = load4(rvel, j << 4);
