@@ -544,6 +544,38 @@ def test_runs_are_deterministic_and_tuning_invariant(E):
         assert np.array_equal(res[0], r)
 
 
+@pytest.mark.parametrize("xsub", [1, 2, 4])
+def test_staged_equals_gathered_on_random_boxes(E, xsub):
+    """The staged passes (each row's candidates copied into LDS; a wave in
+    one y/z column bounds the density pass's copy by two reads of the
+    cell-start table) against the gathered ones, bit for bit, on boxes of
+    several shapes whose particles crowd the grid's first and last cells
+    (the table's first and last entries) at a few times rest density."""
+    rng = np.random.default_rng(100 + xsub)
+    for box in ((0.31, 0.47, 0.23), (0.5, 0.2, 0.35), (0.12, 0.6, 0.6)):
+        bmax = np.array(box)
+        pos = np.concatenate([
+            rng.uniform(0.0, 1.0, size=(12000, 3)) * bmax,
+            bmax - rng.uniform(0.0, 0.08, size=(3000, 3)),
+            rng.uniform(0.0, 0.08, size=(1500, 3)),
+        ]).astype(np.float32)
+        say(f"random box {box} xsub {xsub}: {len(pos)} particles")
+        params = E.make_params(box_max=box)
+        out = []
+        for staged in (0, 1):
+            e = E.Engine(params)
+            e.set_particles(pos)
+            e.set_cell_subdivision(xsub)
+            e.make_ready()
+            e.set_tuning(256, 256, 256, 8, 0, staged, staged)
+            e.advance_steps(4)
+            out.append((e.get_particles(), e.get_velocity(), e.get_density()))
+            e.close()
+        assert all(np.isfinite(x).all() for x in out[0]), (box, xsub)
+        for a, b in zip(*out):
+            assert np.array_equal(a, b), (box, xsub)
+
+
 def test_persistent_launch_matches_one_block_per_tile(E):
     """Density and force as persistent blocks over per-XCD tile queues give
     the same bits as one block per tile (1M particles: more tiles than
