@@ -96,9 +96,10 @@ def plain(args):
     download_ms = (time.perf_counter() - t0) * 1e3
     s, en = e.debug_cells()
     occ = int((en > 0).sum())
-    # Per step, the survey's compulsory traffic: hash (its key is computed in
-    # the force pass, but the bytes are the pass's), sort, reorder, density,
-    # force + integrate, plus 8 bytes per occupied cell: 208 + 8 C_occ / N.
+    # Per step, the compulsory traffic of each pass: hash (its key is computed
+    # in the force pass, but the bytes are the pass's), sort, reorder,
+    # density, force + integrate, plus 8 bytes per occupied cell (its start
+    # and end in the cell table): 208 + 8 C_occ / N.
     b_step = sum(B_ALG.values()) + 8.0 * occ / n
     dom = max(("sort", "reorder", "density", "force"), key=lambda k: per[k])
     dom_bytes = B_ALG[dom] * n + (8.0 * occ if dom == "reorder" else 0.0)
